@@ -1,0 +1,119 @@
+/*
+ * mach_hip.h - C ABI of the MI355X (gfx950) delay-and-sum beamformer.
+ *
+ * Every pointer argument named as "device" must point to device memory of the current HIP device.
+ * Arrays are C-contiguous. Coordinates are float32 [n][3] (x, y, z in metres). Complex data is
+ * interleaved (re, im) float32 (complex64). FP16 IQ data is interleaved (re, im) IEEE binary16.
+ * Scalars are float32, exactly the type the reference binding passes to its kernel.
+ * Functions return MACH_OK (0) or a mach_status code; mach_last_error() then describes the failure
+ * (thread-local; valid until the next call on the same thread). No function calls exit().
+ * Work is enqueued on `stream` (a hipStream_t; NULL = the default stream) and is not synchronised.
+ */
+#ifndef MACH_HIP_H
+#define MACH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MACH_ABI_VERSION 1
+
+/* Interpolation selector; same integer values as the reference's
+ * `enum class InterpolationType` (src/mach/kernel.cu:25-29, exported at kernel.cu:1222-1226). */
+typedef enum mach_interp_type {
+    MACH_INTERP_NEAREST = 0,
+    MACH_INTERP_LINEAR = 1,
+    MACH_INTERP_QUADRATIC = 2
+} mach_interp_type;
+
+/* Storage type of channel data and the matching output type. The reference supports float (RF)
+ * and float2 (IQ) only (src/mach/kernel.cu:543-545); FP16 IQ storage is new here. */
+typedef enum mach_data_type {
+    MACH_DATA_F32 = 0,   /* RF float32 in, float32 out                       */
+    MACH_DATA_C64 = 1,   /* IQ complex64 in, complex64 out                   */
+    MACH_DATA_C32H = 2   /* IQ as (binary16 re, binary16 im) in, complex64 out */
+} mach_data_type;
+
+typedef enum mach_status {
+    MACH_OK = 0,
+    MACH_ERR_INVALID_ARGUMENT = 1, /* reference: std::runtime_error (kernel.cu:740-769)       */
+    MACH_ERR_DIMENSION = 2,        /* reference: "Dimension mismatch ..." (kernel.cu:914-951) */
+    MACH_ERR_HIP = 3,              /* reference: checkCudaErrors -> exit(1); here a status    */
+    MACH_ERR_NO_DEVICE = 4,
+    MACH_ERR_UNSUPPORTED = 5
+} mach_status;
+
+/* General beamform request. Replaces, in one call, the reference's per-transmit loop
+ * `for t: kernel.beamform(channel_data[t], ..., tx_wave_arrivals_s[t], out=out)`
+ * (src/mach/experimental.py:80-97) on top of `beamform<T>` (src/mach/kernel.cu:1033-1209):
+ * transmits are compounded inside the kernel instead of by n_tx launches. */
+typedef struct mach_beamform_args {
+    const void *channel_data;         /* device [n_tx][n_rx][n_samples][n_frames] of data_type  */
+    const float *rx_coords_m;         /* device [n_rx][3]                                       */
+    const float *scan_coords_m;       /* device [n_vox][3]                                      */
+    const float *tx_wave_arrivals_s;  /* device [n_tx][n_vox]                                   */
+    void *out;                        /* device [n_vox][n_frames]; complex64 for IQ, f32 for RF */
+    uint64_t n_tx;
+    uint64_t n_rx;
+    uint64_t n_samples;
+    uint64_t n_frames;
+    uint64_t n_vox;
+    int32_t data_type;                /* mach_data_type                                         */
+    int32_t interp_type;              /* mach_interp_type                                       */
+    float f_number;
+    float rx_start_s;
+    float sampling_freq_hz;
+    float sound_speed_m_s;
+    float modulation_freq_hz;         /* ignored for RF                                         */
+    float tukey_alpha;                /* [0, 1]; 0 disables apodization                         */
+    int32_t accumulate;               /* 1: out += result (reference device-array semantics,
+                                         kernel.cu:674-680); 0: out = result                    */
+    int32_t reserved0;
+} mach_beamform_args;
+
+/* Complex64 (IQ) entry. Replaces the nanobind overload `_cuda_impl.beamform` for complex64
+ * (src/mach/kernel.cu:1229-1242, implemented by beamform<std::complex<float>> at 1033-1209 and
+ * _beamform_impl at 715-891): same arguments in the same order, plus explicit sizes and a stream.
+ * `out` is accumulated into, as the reference does for device arrays. */
+int mach_beamform_c64(const float *channel_data, const float *rx_coords_m, const float *scan_coords_m,
+                      const float *tx_wave_arrivals_s, float *out, uint64_t n_rx, uint64_t n_samples,
+                      uint64_t n_frames, uint64_t n_vox, float f_number, float rx_start_s,
+                      float sampling_freq_hz, float sound_speed_m_s, float modulation_freq_hz,
+                      float tukey_alpha, int32_t interp_type, void *stream);
+
+/* Float32 (RF) entry. Replaces the nanobind overload `_cuda_impl.beamform` for float32
+ * (src/mach/kernel.cu:1244-1255, beamform<float>). `out` is accumulated into. */
+int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const float *scan_coords_m,
+                      const float *tx_wave_arrivals_s, float *out, uint64_t n_rx, uint64_t n_samples,
+                      uint64_t n_frames, uint64_t n_vox, float f_number, float rx_start_s,
+                      float sampling_freq_hz, float sound_speed_m_s, float modulation_freq_hz,
+                      float tukey_alpha, int32_t interp_type, void *stream);
+
+/* General entry (transmit compounding, FP16 IQ storage, overwrite or accumulate). Replaces
+ * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209. */
+int mach_beamform(const mach_beamform_args *args, void *stream);
+
+/* Message of the last failure on this thread ("" if none). Replaces the text of the reference's
+ * std::runtime_error messages (src/mach/kernel.cu:740-769, 926-950). */
+const char *mach_last_error(void);
+
+/* ABI version (MACH_ABI_VERSION). */
+int32_t mach_abi_version(void);
+
+/* HIP runtime version as "major.minor.patch". Replaces the module attribute __nvcc_version__
+ * (src/mach/kernel.cu:1215). */
+const char *mach_hip_runtime_version(void);
+
+/* Writes the current device's gcnArchName into `arch` (NUL-terminated, truncated to `len`) and
+ * returns MACH_OK, or MACH_ERR_NO_DEVICE. Replaces the import-time capability checks
+ * checkCudaDriverCompatibility / checkComputeCapability (src/mach/kernel.cu:357-423). */
+int mach_device_arch(char *arch, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACH_HIP_H */

@@ -1,0 +1,279 @@
+"""Native entry points of the HIP beamformer: the drop-in for the reference's ``mach._cuda_impl``.
+
+The reference exposes a nanobind module ``_cuda_impl`` with ``beamform`` (two overloads, complex64
+and float32, ``src/mach/kernel.cu:1229-1255``; stub ``src/mach/_cuda_impl.pyi:24-53``),
+``InterpolationType`` (``kernel.cu:1222-1226``) and ``__nvcc_version__`` (``kernel.cu:1215``). This
+module exposes the same names over the C ABI of ``include/mach_hip.h`` (ctypes), with
+``__hip_version__`` in place of ``__nvcc_version__``.
+
+Arrays are taken through DLPack (PyTorch tensors directly, NumPy arrays via ``torch.from_numpy``,
+anything else via ``torch.from_dlpack``); ROCm arrays (DLPack device type 10) are used in place.
+CPU arrays are staged to the GPU with the reference's latency warning (``kernel.cu:1092-1100``); a
+CPU ``out`` is overwritten, a device ``out`` is accumulated into (``kernel.cu:1135-1139, 1203-1205,
+674-680``). Without a HIP device this module raises: there is no CPU implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_LIB = _lib.load()
+
+__hip_version__: str = _LIB.mach_hip_runtime_version().decode()
+
+
+class InterpolationType(enum.IntEnum):
+    """Interpolation selector with the reference's values (``src/mach/kernel.cu:25-29``)."""
+
+    NearestNeighbor = 0
+    Linear = 1
+    Quadratic = 2
+
+
+NearestNeighbor = InterpolationType.NearestNeighbor
+Linear = InterpolationType.Linear
+Quadratic = InterpolationType.Quadratic
+
+
+def _shape_str(t) -> str:
+    return "[" + ", ".join(str(int(d)) for d in t.shape) + "]"
+
+
+def _as_tensor(x, name: str) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x
+    if isinstance(x, np.ndarray):
+        with warnings.catch_warnings():  # read-only NumPy inputs are only read
+            warnings.simplefilter("ignore")
+            return torch.from_numpy(x)
+    if hasattr(x, "__dlpack__"):
+        return torch.from_dlpack(x)
+    raise TypeError(f"Array '{name}' does not support DLPack protocol")
+
+
+def _data_kind(cd: torch.Tensor) -> int:
+    if cd.dtype == torch.complex64:
+        return _lib.DATA_C64
+    if cd.dtype == torch.float32:
+        return _lib.DATA_F32
+    if cd.dtype == torch.complex32:
+        return _lib.DATA_C32H
+    if cd.dtype == torch.float16 and cd.shape[-1] == 2:
+        return _lib.DATA_C32H
+    raise TypeError(
+        "beamform(): incompatible channel_data dtype "
+        f"{cd.dtype}; expected complex64 (IQ), float32 (RF), or float16 [..., 2] / complex32 (FP16 IQ)"
+    )
+
+
+def _core_shape(cd: torch.Tensor, kind: int):
+    """Logical [.., n_rx, n_samples, n_frames] shape (drops the trailing FP16 IQ axis)."""
+    if kind == _lib.DATA_C32H and cd.dtype == torch.float16:
+        return tuple(cd.shape[:-1])
+    return tuple(cd.shape)
+
+
+def check_dimensions(channel_shape, rx, scan, arrivals, out, n_rx, n_vox, n_frames):
+    """Shape checks with the reference's messages (``src/mach/kernel.cu:914-951``)."""
+
+    def s(shape):
+        return "[" + ", ".join(str(int(d)) for d in shape) + "]"
+
+    if n_rx != channel_shape[-3] or n_rx != rx.shape[0]:
+        raise RuntimeError(
+            "Dimension mismatch in receive elements:\n"
+            f"  Expected n_receive_elements: {n_rx}\n"
+            f"  channel_data.shape: {s(channel_shape)}\n"
+            f"  rx_coords_m.shape: {_shape_str(rx)}\n"
+            "→ channel_data.shape[0] and rx_coords_m.shape[0] must both equal n_receive_elements"
+        )
+    if n_vox != arrivals.shape[-1] or n_vox != scan.shape[0] or n_vox != out.shape[0]:
+        raise RuntimeError(
+            "Dimension mismatch in output voxels:\n"
+            f"  Expected n_output_voxels: {n_vox}\n"
+            f"  scan_coords_m.shape: {_shape_str(scan)}\n"
+            f"  tx_wave_arrivals_s.shape: {_shape_str(arrivals)}\n"
+            f"  out.shape: {_shape_str(out)}\n"
+            "→ scan_coords_m.shape[0], tx_wave_arrivals_s.shape[0], and out.shape[0] must all equal n_output_voxels"
+        )
+    if n_frames != channel_shape[-1] or n_frames != out.shape[1]:
+        raise RuntimeError(
+            "Dimension mismatch in frames:\n"
+            f"  Expected n_frames: {n_frames}\n"
+            f"  channel_data.shape: {s(channel_shape)}\n"
+            f"  out.shape: {_shape_str(out)}\n"
+            "→ channel_data.shape[2] and out.shape[1] must both equal n_frames"
+        )
+
+
+def _require(t: torch.Tensor, name: str, dtype, ndim: int, last=None):
+    if t.dtype != dtype:
+        raise TypeError(f"beamform(): incompatible {name} dtype {t.dtype}; expected {dtype}")
+    if t.dim() != ndim or (last is not None and t.shape[-1] != last):
+        raise TypeError(f"beamform(): incompatible {name} shape {_shape_str(t)}")
+    if not t.is_contiguous():
+        raise TypeError(f"beamform(): {name} must be C-contiguous")
+
+
+def _check_writable(out):
+    if isinstance(out, np.ndarray) and not out.flags.writeable:
+        raise ValueError("Output array `out` is not writable.")
+
+
+def _device_for(tensors):
+    for t in tensors:
+        if t.is_cuda:
+            return t.device
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "No HIP device is available: mach_amd beamforming runs only on a ROCm GPU (gfx950); "
+            "there is no CPU implementation."
+        )
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
+         sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate_device_out=True):
+    """Stage CPU arrays if needed, call ``mach_beamform`` and write ``out`` back."""
+    arrays = [cd, rx, scan, arrivals, out]
+    n_cpu = sum(1 for t in arrays if not t.is_cuda)
+    for t in arrays:
+        if t.device.type not in ("cpu", "cuda"):
+            raise RuntimeError(
+                f"Found input array on device: {t.device}. Only CPU and ROCm devices are supported."
+            )
+    device = _device_for(arrays)
+    if n_cpu:
+        warnings.warn(
+            f"Found {n_cpu} input array(s) on CPU. This will add latency due to CPU<->GPU memory "
+            "transfers. For optimal performance with HIP beamforming, move arrays to the GPU "
+            "(e.g. PyTorch tensors on a ROCm device).",
+            UserWarning,
+            stacklevel=3,
+        )
+    on_dev = [t if t.is_cuda else t.to(device) for t in arrays[:4]]
+    if any(t.is_cuda and t.device != device for t in arrays):
+        raise RuntimeError("All GPU arrays must be on the same device")
+    host_out = not out.is_cuda
+    dev_out = torch.zeros(out.shape, dtype=out.dtype, device=device) if host_out else out
+    n_rx = on_dev[1].shape[0]
+    n_vox = on_dev[2].shape[0]
+    n_frames = dev_out.shape[1]
+    core = _core_shape(on_dev[0], kind)
+    args = _lib.BeamformArgs(
+        on_dev[0].data_ptr(), on_dev[1].data_ptr(), on_dev[2].data_ptr(), on_dev[3].data_ptr(),
+        dev_out.data_ptr(),
+        n_tx, n_rx, core[-2], n_frames, n_vox,
+        kind, int(interp_type),
+        f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s, modulation_freq_hz, tukey_alpha,
+        1 if (accumulate_device_out or host_out) else 0, 0,
+    )
+    with torch.cuda.device(device):
+        stream = torch.cuda.current_stream(device).cuda_stream
+        status = _LIB.mach_beamform(ctypes.byref(args), ctypes.c_void_p(stream))
+        _lib.check(status)
+        if host_out:
+            out.copy_(dev_out.cpu())
+
+
+def beamform(
+    channel_data,
+    rx_coords_m,
+    scan_coords_m,
+    tx_wave_arrivals_s,
+    out,
+    f_number: float,
+    rx_start_s: float,
+    sampling_freq_hz: float,
+    sound_speed_m_s: float,
+    modulation_freq_hz: float | None = None,
+    tukey_alpha: float = 0.5,
+    interp_type: InterpolationType = InterpolationType.Linear,
+) -> None:
+    """Raw single-transmit beamform, the drop-in for ``_cuda_impl.beamform``.
+
+    Strict like the nanobind overloads (``.noconvert()``, ``kernel.cu:1229-1255``): channel data must
+    be complex64 [n_rx, n_samples, n_frames] (``modulation_freq_hz`` required) or float32 (defaults to
+    0.0); FP16 IQ storage (float16 [n_rx, n_samples, n_frames, 2] or complex32) is also accepted,
+    with a complex64 ``out``. Coordinates are float32 [n, 3], arrivals float32 [n_vox], ``out``
+    [n_vox, n_frames]. Arrays must be C-contiguous.
+    """
+    cd = _as_tensor(channel_data, "channel_data")
+    kind = _data_kind(cd)
+    if kind != _lib.DATA_F32 and modulation_freq_hz is None:
+        raise TypeError("beamform(): modulation_freq_hz is required for complex channel data")
+    if modulation_freq_hz is None:
+        modulation_freq_hz = 0.0
+    rx = _as_tensor(rx_coords_m, "rx_coords_m")
+    scan = _as_tensor(scan_coords_m, "scan_coords_m")
+    arr = _as_tensor(tx_wave_arrivals_s, "tx_wave_arrivals_s")
+    o = _as_tensor(out, "out")
+    core_ndim = 4 if (kind == _lib.DATA_C32H and cd.dtype == torch.float16) else 3
+    if cd.dim() != core_ndim or not cd.is_contiguous():
+        raise TypeError(f"beamform(): incompatible channel_data shape {_shape_str(cd)} (C-contiguous 3-D expected)")
+    _require(rx, "rx_coords_m", torch.float32, 2, 3)
+    _require(scan, "scan_coords_m", torch.float32, 2, 3)
+    _require(arr, "tx_wave_arrivals_s", torch.float32, 1)
+    _require(o, "out", torch.float32 if kind == _lib.DATA_F32 else torch.complex64, 2)
+    core = _core_shape(cd, kind)
+    check_dimensions(core, rx, scan, arr, o, rx.shape[0], scan.shape[0], o.shape[1])
+    _check_writable(out)
+    _run(cd, rx, scan, arr, o, n_tx=1, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
+         sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
+         modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type)
+
+
+def beamform_compound(
+    channel_data,
+    rx_coords_m,
+    scan_coords_m,
+    tx_wave_arrivals_s,
+    out,
+    f_number: float,
+    rx_start_s: float,
+    sampling_freq_hz: float,
+    sound_speed_m_s: float,
+    modulation_freq_hz: float | None = None,
+    tukey_alpha: float = 0.5,
+    interp_type: InterpolationType = InterpolationType.Linear,
+    accumulate: bool = True,
+) -> None:
+    """Coherent compounding in one launch: channel data [n_tx, n_rx, n_samples, n_frames], arrivals
+    [n_tx, n_vox]. Replaces the per-transmit loop of ``src/mach/experimental.py:80-97``.
+    ``accumulate=False`` overwrites a device ``out`` instead of adding to it."""
+    cd = _as_tensor(channel_data, "channel_data")
+    kind = _data_kind(cd)
+    if kind != _lib.DATA_F32 and modulation_freq_hz is None:
+        raise TypeError("beamform_compound(): modulation_freq_hz is required for complex channel data")
+    if modulation_freq_hz is None:
+        modulation_freq_hz = 0.0
+    rx = _as_tensor(rx_coords_m, "rx_coords_m")
+    scan = _as_tensor(scan_coords_m, "scan_coords_m")
+    arr = _as_tensor(tx_wave_arrivals_s, "tx_wave_arrivals_s")
+    o = _as_tensor(out, "out")
+    core_ndim = 5 if (kind == _lib.DATA_C32H and cd.dtype == torch.float16) else 4
+    if cd.dim() != core_ndim or not cd.is_contiguous():
+        raise TypeError(f"beamform_compound(): incompatible channel_data shape {_shape_str(cd)}")
+    _require(rx, "rx_coords_m", torch.float32, 2, 3)
+    _require(scan, "scan_coords_m", torch.float32, 2, 3)
+    _require(arr, "tx_wave_arrivals_s", torch.float32, 2)
+    _require(o, "out", torch.float32 if kind == _lib.DATA_F32 else torch.complex64, 2)
+    core = _core_shape(cd, kind)
+    if arr.shape[0] != core[0]:
+        raise RuntimeError(
+            f"Dimension mismatch in transmits: channel_data.shape {list(core)} vs "
+            f"tx_wave_arrivals_s.shape {_shape_str(arr)}"
+        )
+    check_dimensions(core, rx, scan, arr, o, rx.shape[0], scan.shape[0], o.shape[1])
+    _check_writable(out)
+    _run(cd, rx, scan, arr, o, n_tx=core[0], kind=kind, f_number=f_number, rx_start_s=rx_start_s,
+         sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
+         modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,
+         accumulate_device_out=accumulate)

@@ -1,0 +1,180 @@
+// C ABI of the beamformer (declared in include/mach_hip.h): argument validation, error reporting and
+// dispatch to the delay-and-sum kernels. Validation messages keep the reference's wording
+// (src/mach/kernel.cu:740-769) so callers that match on them keep working.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <climits>
+#include <cstring>
+#include <string>
+
+#include "../../include/mach_hip.h"
+#include "das_kernel.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(MACH_ERR_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorName(e) + " (" +
+                                  hipGetErrorString(e) + ")");
+}
+
+int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
+    if (a == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: args is NULL");
+    if (a->data_type < MACH_DATA_F32 || a->data_type > MACH_DATA_C32H)
+        return fail(MACH_ERR_UNSUPPORTED, "Error: unsupported data_type " + std::to_string(a->data_type));
+    if (a->interp_type < MACH_INTERP_NEAREST || a->interp_type > MACH_INTERP_QUADRATIC)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: unsupported interp_type " + std::to_string(a->interp_type));
+    // kernel.cu:745-748
+    if (!(a->tukey_alpha >= 0.0f && a->tukey_alpha <= 1.0f))
+        return fail(MACH_ERR_INVALID_ARGUMENT,
+                    "Error: tukey_alpha must be in range [0, 1], but got " + std::to_string(a->tukey_alpha));
+    if (a->n_samples > (uint64_t)INT32_MAX)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: n_samples exceeds " + std::to_string(INT32_MAX));
+    if (a->n_tx > (uint64_t)INT64_MAX / 4 || a->n_rx > (uint64_t)INT32_MAX || a->n_vox > (uint64_t)INT64_MAX / 64 ||
+        a->n_frames > (uint64_t)INT32_MAX)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: array dimensions exceed the supported range");
+    // 64-bit element offsets replace the reference's 32-bit indexing cap (kernel.cu:738-743); the
+    // product still has to fit int64.
+    const long double total = (long double)a->n_tx * a->n_rx * a->n_samples * a->n_frames;
+    if (total > (long double)INT64_MAX / 4)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: channel data size exceeds the 64-bit indexing limit");
+    const bool need_data = a->n_vox > 0 && a->n_frames > 0;
+    if (need_data && a->out == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: out is NULL");
+    if (need_data && a->scan_coords_m == nullptr)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: scan_coords_m is NULL");
+    const bool need_inputs = need_data && a->n_tx > 0 && a->n_rx > 0;
+    if (need_inputs && (a->rx_coords_m == nullptr || a->tx_wave_arrivals_s == nullptr))
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: rx_coords_m / tx_wave_arrivals_s is NULL");
+    if (need_inputs && a->n_samples > 0 && a->channel_data == nullptr)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: channel_data is NULL");
+
+    p->data = a->channel_data;
+    p->rx = a->rx_coords_m;
+    p->vox = a->scan_coords_m;
+    p->arrivals = a->tx_wave_arrivals_s;
+    p->out = a->out;
+    p->n_tx = (int64_t)a->n_tx;
+    p->n_rx = (int64_t)a->n_rx;
+    p->n_samples = (int64_t)a->n_samples;
+    p->n_frames = (int64_t)a->n_frames;
+    p->n_vox = (int64_t)a->n_vox;
+    p->tap_step = a->n_samples > 1 ? (int64_t)a->n_frames : 0;
+    p->two_f_number = 2.0 * (double)a->f_number;
+    p->inv_c = 1.0 / (double)a->sound_speed_m_s;
+    p->t0 = (double)a->rx_start_s;
+    p->fs = (double)a->sampling_freq_hz;
+    p->fmod = (double)a->modulation_freq_hz;
+    p->alpha = (double)a->tukey_alpha;
+    p->half_pi_over_alpha = a->tukey_alpha > 0.0f ? (M_PI / 2.0) / (double)a->tukey_alpha : 0.0;
+    p->rotate = (a->data_type != MACH_DATA_F32 && a->modulation_freq_hz != 0.0f) ? 1 : 0;
+    p->accumulate = a->accumulate ? 1 : 0;
+    p->apod = a->tukey_alpha > 0.0f ? 1 : 0;  // kernel.cu:749
+    p->pad0 = 0;
+    return MACH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mach_beamform(const mach_beamform_args *args, void *stream) {
+    g_last_error.clear();
+    mach::DasParams p;
+    const int st = validate_and_build(args, &p);
+    if (st != MACH_OK) return st;
+    if (p.n_vox == 0 || p.n_frames == 0) return MACH_OK;
+    const mach::DasLayout layout = mach::choose_layout(p, args->data_type);
+    const hipError_t e = mach::launch_das(p, args->data_type, args->interp_type, layout, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "beamform kernel launch");
+    return MACH_OK;
+}
+
+static int beamform_single(int data_type, const float *channel_data, const float *rx_coords_m,
+                           const float *scan_coords_m, const float *tx_wave_arrivals_s, float *out, uint64_t n_rx,
+                           uint64_t n_samples, uint64_t n_frames, uint64_t n_vox, float f_number, float rx_start_s,
+                           float sampling_freq_hz, float sound_speed_m_s, float modulation_freq_hz,
+                           float tukey_alpha, int32_t interp_type, void *stream) {
+    mach_beamform_args a;
+    std::memset(&a, 0, sizeof(a));
+    a.channel_data = channel_data;
+    a.rx_coords_m = rx_coords_m;
+    a.scan_coords_m = scan_coords_m;
+    a.tx_wave_arrivals_s = tx_wave_arrivals_s;
+    a.out = out;
+    a.n_tx = 1;
+    a.n_rx = n_rx;
+    a.n_samples = n_samples;
+    a.n_frames = n_frames;
+    a.n_vox = n_vox;
+    a.data_type = data_type;
+    a.interp_type = interp_type;
+    a.f_number = f_number;
+    a.rx_start_s = rx_start_s;
+    a.sampling_freq_hz = sampling_freq_hz;
+    a.sound_speed_m_s = sound_speed_m_s;
+    a.modulation_freq_hz = modulation_freq_hz;
+    a.tukey_alpha = tukey_alpha;
+    a.accumulate = 1;
+    return mach_beamform(&a, stream);
+}
+
+int mach_beamform_c64(const float *channel_data, const float *rx_coords_m, const float *scan_coords_m,
+                      const float *tx_wave_arrivals_s, float *out, uint64_t n_rx, uint64_t n_samples,
+                      uint64_t n_frames, uint64_t n_vox, float f_number, float rx_start_s, float sampling_freq_hz,
+                      float sound_speed_m_s, float modulation_freq_hz, float tukey_alpha, int32_t interp_type,
+                      void *stream) {
+    return beamform_single(MACH_DATA_C64, channel_data, rx_coords_m, scan_coords_m, tx_wave_arrivals_s, out, n_rx,
+                           n_samples, n_frames, n_vox, f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s,
+                           modulation_freq_hz, tukey_alpha, interp_type, stream);
+}
+
+int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const float *scan_coords_m,
+                      const float *tx_wave_arrivals_s, float *out, uint64_t n_rx, uint64_t n_samples,
+                      uint64_t n_frames, uint64_t n_vox, float f_number, float rx_start_s, float sampling_freq_hz,
+                      float sound_speed_m_s, float modulation_freq_hz, float tukey_alpha, int32_t interp_type,
+                      void *stream) {
+    return beamform_single(MACH_DATA_F32, channel_data, rx_coords_m, scan_coords_m, tx_wave_arrivals_s, out, n_rx,
+                           n_samples, n_frames, n_vox, f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s,
+                           modulation_freq_hz, tukey_alpha, interp_type, stream);
+}
+
+const char *mach_last_error(void) { return g_last_error.c_str(); }
+
+int32_t mach_abi_version(void) { return MACH_ABI_VERSION; }
+
+const char *mach_hip_runtime_version(void) {
+    static thread_local char buf[64];
+    int v = 0;
+    if (hipRuntimeGetVersion(&v) != hipSuccess) {
+        std::snprintf(buf, sizeof(buf), "%d.%d.%d", HIP_VERSION_MAJOR, HIP_VERSION_MINOR, HIP_VERSION_PATCH);
+        return buf;
+    }
+    // HIP encodes the runtime version as major * 10000000 + minor * 100000 + patch
+    std::snprintf(buf, sizeof(buf), "%d.%d.%d", v / 10000000, (v / 100000) % 100, v % 100000);
+    return buf;
+}
+
+int mach_device_arch(char *arch, size_t len) {
+    g_last_error.clear();
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(MACH_ERR_NO_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return fail(MACH_ERR_NO_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (arch != nullptr && len > 0) {
+        std::strncpy(arch, prop.gcnArchName, len - 1);
+        arch[len - 1] = '\0';
+    }
+    return MACH_OK;
+}
+
+}  // extern "C"

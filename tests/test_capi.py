@@ -1,0 +1,100 @@
+"""C-ABI library checks that need no GPU: it loads, exports every function include/*.h declares, and
+rejects invalid arguments with the reference's messages before touching the device."""
+
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from mach_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        with open(hdr) as fh:
+            text = fh.read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(mach_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for required in ("mach_beamform", "mach_beamform_c64", "mach_beamform_f32", "mach_last_error",
+                     "mach_abi_version", "mach_hip_runtime_version", "mach_device_arch"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+    for name in declared_functions():
+        assert name in exported, name
+        assert getattr(lib, name) is not None
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature in mach_amd/_lib.py"
+
+
+def test_abi_and_version():
+    lib = _lib.load()
+    assert lib.mach_abi_version() == 1
+    v = lib.mach_hip_runtime_version().decode()
+    major, minor, _ = v.split(".")
+    assert int(major) >= 6 and int(minor) >= 0
+
+
+def _args(**over):
+    a = _lib.BeamformArgs()
+    a.n_tx, a.n_rx, a.n_samples, a.n_frames, a.n_vox = 1, 4, 100, 2, 8
+    a.data_type, a.interp_type = _lib.DATA_C64, 1
+    a.f_number, a.sampling_freq_hz, a.sound_speed_m_s, a.tukey_alpha = 1.0, 1e6, 1500.0, 0.5
+    a.accumulate = 1
+    for k, v in over.items():
+        setattr(a, k, v)
+    return a
+
+
+@pytest.mark.parametrize("alpha", [-0.1, 1.5, float("nan")])
+def test_tukey_alpha_range_is_rejected(alpha):
+    lib = _lib.load()
+    st = lib.mach_beamform(ctypes.byref(_args(tukey_alpha=alpha)), None)
+    assert st == 1
+    assert _lib.last_error().startswith("Error: tukey_alpha must be in range [0, 1], but got")
+
+
+def test_bad_codes_and_null_pointers_are_rejected():
+    lib = _lib.load()
+    assert lib.mach_beamform(None, None) == 1
+    assert lib.mach_beamform(ctypes.byref(_args(data_type=7)), None) == 5
+    assert "unsupported data_type" in _lib.last_error()
+    assert lib.mach_beamform(ctypes.byref(_args(interp_type=3)), None) == 1
+    assert "unsupported interp_type" in _lib.last_error()
+    assert lib.mach_beamform(ctypes.byref(_args()), None) == 1  # out is NULL
+    assert "out is NULL" in _lib.last_error()
+    assert lib.mach_beamform(ctypes.byref(_args(n_samples=1 << 40)), None) == 1
+    with pytest.raises(RuntimeError, match="tukey_alpha"):
+        _lib.check(lib.mach_beamform(ctypes.byref(_args(tukey_alpha=2.0)), None))
+
+
+def test_empty_problem_is_a_no_op():
+    lib = _lib.load()
+    assert lib.mach_beamform(ctypes.byref(_args(n_vox=0)), None) == 0
+    assert lib.mach_beamform(ctypes.byref(_args(n_frames=0)), None) == 0
+
+
+def test_device_query_reports_status():
+    lib = _lib.load()
+    buf = ctypes.create_string_buffer(64)
+    st = lib.mach_device_arch(buf, 64)
+    assert st in (0, 4)
+    if st == 0:
+        assert buf.value.decode().startswith("gfx")
+    else:
+        assert "no HIP device" in _lib.last_error()
