@@ -75,29 +75,73 @@ def run_gpu(acq, storage, steps, warmup):
     return wall / steps * 1e3, kernel_ms, out
 
 
-def cpu_baseline(acq, n_voxels, seed=0):
+def cpu_baseline(acq, n_voxels, min_seconds=10.0, seed=0):
+    """Times the C restatement on host cores over a seeded voxel sample, repeated until at least
+    ``min_seconds`` of CPU work has run; reports the median rate over the repetitions."""
     from oracle.das_c import das_c, max_threads
 
     V = acq.n_vox
-    idx = np.sort(np.random.default_rng(seed).choice(V, min(n_voxels, V), replace=False))
+    n = min(n_voxels, V) if n_voxels > 0 else V
+    idx = np.sort(np.random.default_rng(seed).choice(V, n, replace=False))
     cd = acq.channel_data.cpu().numpy()
     rx = acq.rx_coords_m.cpu().numpy()
-    scan = acq.scan_coords_m.cpu().numpy()[idx]
-    arr = acq.tx_wave_arrivals_s.cpu().numpy()[:, idx]
+    scan = np.ascontiguousarray(acq.scan_coords_m.cpu().numpy()[idx])
+    arr = np.ascontiguousarray(acq.tx_wave_arrivals_s.cpu().numpy()[:, idx])
     threads = len(os.sched_getaffinity(0))
     das_c(cd, rx, scan[:8], arr[:, :8], **acq.params, n_threads=threads)  # warm-up
-    t0 = time.perf_counter()
-    das_c(cd, rx, scan, arr, **acq.params, n_threads=threads)
-    dt = time.perf_counter() - t0
     pts = len(idx) * acq.n_rx * acq.n_frames * acq.n_tx
+    rates, total = [], 0.0
+    while total < min_seconds or len(rates) < 3:
+        t0 = time.perf_counter()
+        das_c(cd, rx, scan, arr, **acq.params, n_threads=threads)
+        dt = time.perf_counter() - t0
+        total += dt
+        rates.append(pts / dt / 1e9)
+        if len(rates) >= 50:
+            break
     return {
-        "value": pts / dt / 1e9,
+        "value": float(np.median(rates)),
         "unit": "Gpts/s",
         "cores": min(threads, max_threads()),
         "kind": "port",
         "sample": f"{len(idx)} of {V} voxels (seeded), all {acq.n_rx} ch x {acq.n_frames} frames x {acq.n_tx} tx; "
-                  f"FP64 C restatement, {dt:.1f} s",
+                  f"FP64 C restatement (oracle/das_oracle.c, OpenMP), median of {len(rates)} runs, "
+                  f"{total:.1f} s total",
     }
+
+
+def served_stream(n_requests, qps, seed=4):
+    """Mixed request stream through the continuous-batching server vs one plain launch per request."""
+    from mach_amd import serving
+
+    dev = torch.device("cuda", 0)
+    reqs = serving.make_stream(dev, n_requests=n_requests, qps=qps, seed=seed)
+    # warm both paths (graph capture, template caches) on a short prefix
+    warm = reqs[:8]
+    serving.serve_plain(warm, dev)
+    serving.serve_batched(warm, dev)
+    plain = serving.serve_plain(reqs, dev)
+    batched = serving.serve_batched(reqs, dev)
+    return {
+        "workload": f"{n_requests} requests, Poisson {qps:g} QPS (seed {seed}), drawn from plane_wave/compound/"
+                    "doppler/volumetric with grid x0.5-1.0 and 1-256 frames (plane_wave, doppler)",
+        "budget_gb": 200,
+        "batched": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in batched.as_dict().items()},
+        "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain.as_dict().items()},
+        "throughput_ratio": round(batched.gpts_per_s / plain.gpts_per_s, 3),
+    }
+
+
+def copy_calibration(mib: int, reps: int = 5):
+    """A plain device-to-device copy of known size, run so that a counter pass over this process can
+    scale its FETCH_SIZE / WRITE_SIZE readings (bytes moved = mib MiB read + mib MiB written)."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def main(argv=None):
@@ -107,8 +151,14 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="doppler", choices=["doppler", "plane_wave", "compound", "volumetric"])
     ap.add_argument("--storage", default=None, choices=["c64", "fp16"])
-    ap.add_argument("--cpu-voxels", type=int, default=1024)
+    ap.add_argument("--cpu-voxels", type=int, default=0, help="voxel sample for the CPU baseline (0 = all)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served mixed-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-qps", type=float, default=300.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run a plain copy of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the delay-and-sum path is "
@@ -121,6 +171,8 @@ def main(argv=None):
     from mach_amd import phantoms
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     storage = args.storage or ("fp16" if args.workload == "volumetric" else "c64")
     log(f"bench: synthesizing {args.workload} (seeded) on {dev}")
     acq = phantoms.make(args.workload, device=dev)
@@ -177,9 +229,14 @@ def main(argv=None):
             "flop_per_point": FLOP_PER_POINT[interp],
         },
     }
+    if not args.no_serve:
+        log("bench: served mixed stream (batched persistent kernel vs plain per-request)")
+        result["served"] = served_stream(args.serve_requests, args.serve_qps)
+        log(f"bench: served {result['served']['throughput_ratio']}x plain; "
+            f"p50/p99 {result['served']['batched']['p50_ms']}/{result['served']['batched']['p99_ms']} ms")
     if not args.no_cpu_baseline:
         log("bench: timing CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(acq, args.cpu_voxels)
+        result["cpu_baseline"] = cpu_baseline(acq, args.cpu_voxels, args.cpu_seconds)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -71,7 +71,15 @@ typedef struct mach_beamform_args {
     float tukey_alpha;                /* [0, 1]; 0 disables apodization                         */
     int32_t accumulate;               /* 1: out += result (reference device-array semantics,
                                          kernel.cu:674-680); 0: out = result                    */
-    int32_t reserved0;
+    int32_t layout;                   /* 0: automatic. Otherwise (lanes_per_voxel << 4) |
+                                         frames_per_lane, e.g. 0x404: 64 lanes share a voxel,
+                                         4 frames each (the autotuner's choice; rejected if it
+                                         does not divide n_frames or break vector alignment)    */
+    const int32_t *voxel_order;       /* optional device [n_vox] permutation of 0..n_vox-1: the
+                                         order voxels are processed in (NULL = 0, 1, 2, ...).
+                                         Results still land at out[v]; a depth-major order
+                                         (sorted by z, then x, then y) keeps the sample windows
+                                         of neighbouring waves in the same L2.                   */
 } mach_beamform_args;
 
 /* Complex64 (IQ) entry. Replaces the nanobind overload `_cuda_impl.beamform` for complex64
@@ -95,6 +103,26 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
 /* General entry (transmit compounding, FP16 IQ storage, overwrite or accumulate). Replaces
  * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209. */
 int mach_beamform(const mach_beamform_args *args, void *stream);
+
+/* Batched execution: many requests (any mix of shapes, data types, interpolation types and
+ * transmit counts) run in ONE persistent-kernel launch that drains a device work queue, replayed
+ * from a captured hipGraph. No reference counterpart: the reference runs one synchronous launch per
+ * request and transmit (src/mach/kernel.cu:820-874, src/mach/experimental.py:80-97).
+ * The batch object owns two descriptor slots, so two submissions can be in flight; a third waits
+ * for the oldest. Requests' arrays must stay valid until the submission's work has completed on
+ * `stream`. Results are identical to mach_beamform on each request alone (same per-voxel order). */
+typedef struct mach_batch mach_batch;
+
+/* Allocates the batch object (pinned + device descriptor tables for up to `max_requests` each).
+ * use_graph != 0 replays a captured hipGraph (copy descriptors -> reset queue -> persistent kernel). */
+int mach_batch_create(uint32_t max_requests, int32_t use_graph, mach_batch **batch);
+
+/* Enqueues all `n_requests` requests on `stream` as one persistent launch. */
+int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uint32_t n_requests,
+                      void *stream);
+
+/* Waits for the batch's pending work and frees it. */
+int mach_batch_destroy(mach_batch *batch);
 
 /* Message of the last failure on this thread ("" if none). Replaces the text of the reference's
  * std::runtime_error messages (src/mach/kernel.cu:740-769, 926-950). */

@@ -15,9 +15,11 @@ CPU ``out`` is overwritten, a device ``out`` is accumulated into (``kernel.cu:11
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import enum
 import warnings
+import weakref
 
 import numpy as np
 import torch
@@ -127,6 +129,40 @@ def _check_writable(out):
         raise ValueError("Output array `out` is not writable.")
 
 
+# Voxels below this count are processed in the caller's order (no reordering benefit).
+_ORDER_MIN_VOXELS = 4096
+_ORDER_CACHE: "collections.OrderedDict" = collections.OrderedDict()
+_ORDER_CACHE_SIZE = 16
+
+
+def voxel_order(scan: torch.Tensor):
+    """Depth-major processing order of the voxels: sorted by z, then x, then y (stable).
+
+    Waves that run together then beamform voxels at the same depth, whose interpolation taps hit the
+    same few channel-data rows per element, so those rows stay in L2. Any permutation gives the same
+    results (each voxel is computed independently, with a fixed summation order); the order only
+    affects speed. Cached per scan-coordinate tensor (identity and in-place version).
+    """
+    n = scan.shape[0]
+    if n < _ORDER_MIN_VOXELS or not scan.is_cuda:
+        return None
+    key = (id(scan), scan.data_ptr(), n)
+    hit = _ORDER_CACHE.get(key)
+    if hit is not None:
+        ref, version, order = hit
+        if ref() is scan and version == scan._version:
+            _ORDER_CACHE.move_to_end(key)
+            return order
+    idx = torch.argsort(scan[:, 1], stable=True)
+    idx = idx[torch.argsort(scan[idx, 0], stable=True)]
+    idx = idx[torch.argsort(scan[idx, 2], stable=True)]
+    order = idx.to(torch.int32).contiguous()
+    _ORDER_CACHE[key] = (weakref.ref(scan), scan._version, order)
+    while len(_ORDER_CACHE) > _ORDER_CACHE_SIZE:
+        _ORDER_CACHE.popitem(last=False)
+    return order
+
+
 def _device_for(tensors):
     for t in tensors:
         if t.is_cuda:
@@ -139,8 +175,23 @@ def _device_for(tensors):
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def make_args(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
+              sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate=True, layout=0,
+              order=None) -> "_lib.BeamformArgs":
+    """``mach_beamform_args`` for device tensors (shapes already checked)."""
+    core = _core_shape(cd, kind)
+    return _lib.BeamformArgs(
+        cd.data_ptr(), rx.data_ptr(), scan.data_ptr(), arrivals.data_ptr(), out.data_ptr(),
+        n_tx, rx.shape[0], core[-2], out.shape[1], scan.shape[0],
+        kind, int(interp_type),
+        f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s, modulation_freq_hz, tukey_alpha,
+        1 if accumulate else 0, int(layout),
+        order.data_ptr() if order is not None else None,
+    )
+
+
 def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
-         sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate_device_out=True):
+         sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate_device_out=True, layout=0):
     """Stage CPU arrays if needed, call ``mach_beamform`` and write ``out`` back."""
     arrays = [cd, rx, scan, arrivals, out]
     n_cpu = sum(1 for t in arrays if not t.is_cuda)
@@ -163,18 +214,17 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
         raise RuntimeError("All GPU arrays must be on the same device")
     host_out = not out.is_cuda
     dev_out = torch.zeros(out.shape, dtype=out.dtype, device=device) if host_out else out
-    n_rx = on_dev[1].shape[0]
-    n_vox = on_dev[2].shape[0]
-    n_frames = dev_out.shape[1]
-    core = _core_shape(on_dev[0], kind)
-    args = _lib.BeamformArgs(
-        on_dev[0].data_ptr(), on_dev[1].data_ptr(), on_dev[2].data_ptr(), on_dev[3].data_ptr(),
-        dev_out.data_ptr(),
-        n_tx, n_rx, core[-2], n_frames, n_vox,
-        kind, int(interp_type),
-        f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s, modulation_freq_hz, tukey_alpha,
-        1 if (accumulate_device_out or host_out) else 0, 0,
-    )
+    order = voxel_order(on_dev[2])
+    if layout == 0:
+        from . import autotune
+
+        core = _core_shape(on_dev[0], kind)
+        layout = autotune.layout_for(kind, int(interp_type), n_tx, core[-3], dev_out.shape[1], on_dev[2].shape[0],
+                                     on_dev[0].data_ptr(), dev_out.data_ptr())
+    args = make_args(*on_dev, dev_out, n_tx=n_tx, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
+                     sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
+                     modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,
+                     accumulate=accumulate_device_out or host_out, layout=layout, order=order)
     with torch.cuda.device(device):
         stream = torch.cuda.current_stream(device).cuda_stream
         status = _LIB.mach_beamform(ctypes.byref(args), ctypes.c_void_p(stream))
@@ -244,10 +294,12 @@ def beamform_compound(
     tukey_alpha: float = 0.5,
     interp_type: InterpolationType = InterpolationType.Linear,
     accumulate: bool = True,
+    layout: int = 0,
 ) -> None:
     """Coherent compounding in one launch: channel data [n_tx, n_rx, n_samples, n_frames], arrivals
     [n_tx, n_vox]. Replaces the per-transmit loop of ``src/mach/experimental.py:80-97``.
-    ``accumulate=False`` overwrites a device ``out`` instead of adding to it."""
+    ``accumulate=False`` overwrites a device ``out`` instead of adding to it; ``layout`` forces a
+    kernel layout ((lanes_per_voxel << 4) | frames_per_lane, 0 = automatic)."""
     cd = _as_tensor(channel_data, "channel_data")
     kind = _data_kind(cd)
     if kind != _lib.DATA_F32 and modulation_freq_hz is None:
@@ -276,4 +328,4 @@ def beamform_compound(
     _run(cd, rx, scan, arr, o, n_tx=core[0], kind=kind, f_number=f_number, rx_start_s=rx_start_s,
          sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
          modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,
-         accumulate_device_out=accumulate)
+         accumulate_device_out=accumulate, layout=layout)

@@ -50,7 +50,8 @@ class BeamformArgs(ctypes.Structure):
         ("modulation_freq_hz", ctypes.c_float),
         ("tukey_alpha", ctypes.c_float),
         ("accumulate", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
+        ("voxel_order", ctypes.c_void_p),
     ]
 
 
@@ -66,6 +67,10 @@ SIGNATURES = {
     "mach_beamform": (ctypes.c_int, [ctypes.POINTER(BeamformArgs), ctypes.c_void_p]),
     "mach_beamform_c64": (ctypes.c_int, _SINGLE_ARGTYPES),
     "mach_beamform_f32": (ctypes.c_int, _SINGLE_ARGTYPES),
+    "mach_batch_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "mach_batch_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BeamformArgs), ctypes.c_uint32,
+                                         ctypes.c_void_p]),
+    "mach_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "mach_last_error": (ctypes.c_char_p, []),
     "mach_abi_version": (ctypes.c_int32, []),
     "mach_hip_runtime_version": (ctypes.c_char_p, []),
@@ -87,7 +92,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise NativeLibraryMissing(
             f"mach_amd native library not found at {LIB_PATH}; build it with "
-            "`python -m mach_amd.build` (hipcc, gfx950). There is no CPU fallback."
+            "`python mach_amd/build.py` (hipcc, gfx950). There is no CPU fallback."
         )
     lib = ctypes.CDLL(LIB_PATH)
     for name, (restype, argtypes) in SIGNATURES.items():

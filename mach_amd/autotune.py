@@ -1,0 +1,190 @@
+"""Autotuner for the delay-and-sum kernel layout.
+
+The kernel's tunable shape is the frame-lane layout: how many lanes of a wave share one voxel
+(``lanes_per_voxel``: the voxel tile per wave is 64 / lanes_per_voxel voxels) and how many
+consecutive frames each lane carries in registers (``frames_per_lane``: the vector width of every
+channel-data load). All layouts give bit-identical results (same per-voxel summation order); they
+differ in speed with the frame count, voxel count and data type.
+
+``tune`` times every valid candidate on a request with HIP events and returns the fastest;
+``TuningTable`` stores winners per shape class in JSON. ``mach_amd`` consults the table at call time
+(``MACH_AMD_TUNING`` names the file; default ``mach_amd/tuning/gfx950.json``), and falls back to the
+built-in heuristic when a shape class has no entry.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import threading
+
+import torch
+
+LANES = (1, 2, 4, 8, 16, 32, 64)
+FRAMES_PER_LANE = (1, 2, 4)
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "gfx950.json")
+
+
+def encode(lanes_per_voxel: int, frames_per_lane: int) -> int:
+    return (lanes_per_voxel << 4) | frames_per_lane
+
+
+def decode(layout: int):
+    return (layout >> 4) & 0xFFF, layout & 0xF
+
+
+def _bucket(n: int) -> int:
+    """Next power of two (shape classes group sizes by octave)."""
+    return 1 << max(0, math.ceil(math.log2(max(1, n))))
+
+
+def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int) -> str:
+    return f"d{kind}:i{int(interp)}:t{_bucket(n_tx)}:e{_bucket(n_rx)}:f{n_frames}:v{_bucket(n_vox)}"
+
+
+def candidates(n_frames: int):
+    """Layouts worth timing for ``n_frames``: frames-per-lane divides the frame count and a voxel's
+    lanes cover no more than twice the frames they have to share."""
+    out = []
+    for k in FRAMES_PER_LANE:
+        if n_frames % k:
+            continue
+        for lf in LANES:
+            if lf * k <= max(2 * n_frames, 1) and (lf * k >= min(n_frames, 64) // 4 or lf == 1):
+                out.append(encode(lf, k))
+    return out
+
+
+def time_layout(run, layout: int, reps: int = 5, warmup: int = 1) -> float:
+    """Average milliseconds of ``run(layout)`` (one launch) over ``reps``, HIP events on the stream."""
+    for _ in range(warmup):
+        run(layout)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        run(layout)
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(run, n_frames: int, reps: int = 5):
+    """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms})."""
+    times = {}
+    for lay in candidates(n_frames):
+        try:
+            times[lay] = time_layout(run, lay, reps)
+        except RuntimeError:  # layout rejected for this request (alignment)
+            continue
+    best = min(times, key=times.get)
+    return best, times
+
+
+class TuningTable:
+    """Shape class -> layout, persisted as JSON."""
+
+    def __init__(self, path: str | None = None):
+        self.path = path or os.environ.get("MACH_AMD_TUNING", DEFAULT_TABLE)
+        self.entries: dict = {}
+        self._lock = threading.Lock()
+        if os.path.exists(self.path):
+            with open(self.path) as fh:
+                data = json.load(fh)
+            self.entries = {k: int(v["layout"]) for k, v in data.get("entries", {}).items()}
+
+    def lookup(self, key: str) -> int:
+        return self.entries.get(key, 0)
+
+    def record(self, key: str, layout: int, times: dict | None = None, meta: dict | None = None) -> None:
+        with self._lock:
+            self.entries[key] = int(layout)
+            self._meta = getattr(self, "_meta", {})
+            self._meta[key] = {"layout": int(layout), "layout_hex": hex(int(layout)),
+                               "times_ms": {hex(k): round(v, 5) for k, v in (times or {}).items()},
+                               **(meta or {})}
+
+    def save(self, path: str | None = None) -> str:
+        path = path or self.path
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        meta = getattr(self, "_meta", {})
+        data = {"arch": "gfx950", "entries": {k: meta.get(k, {"layout": v, "layout_hex": hex(v)})
+                                              for k, v in sorted(self.entries.items())}}
+        with open(path, "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        return path
+
+
+_TABLE = None
+
+
+def table() -> TuningTable:
+    global _TABLE
+    if _TABLE is None:
+        _TABLE = TuningTable()
+    return _TABLE
+
+
+def layout_for(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int,
+               data_ptr: int = 0, out_ptr: int = 0) -> int:
+    """Tuned layout for a request, or 0 (let the library choose) when the shape class has no entry or
+    the tuned frames-per-lane does not fit this call's pointer alignment."""
+    lay = table().lookup(shape_key(kind, interp, n_tx, n_rx, n_frames, n_vox))
+    if lay:
+        _, k = decode(lay)
+        elem = 8 if kind == 1 else 4
+        out_elem = 4 if kind == 0 else 8
+        if n_frames % k or data_ptr % (k * elem) or out_ptr % (k * out_elem):
+            return 0
+    return lay
+
+
+def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = True):
+    """Tunes the layout for a synthetic acquisition (``mach_amd.phantoms.Acquisition``)."""
+    from . import _hip_impl
+
+    p = acq.params
+    cd = acq.fp16_channel_data() if storage == "fp16" else acq.channel_data
+    kind = _hip_impl._data_kind(cd)
+    out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
+
+    def run(layout):
+        _hip_impl.beamform_compound(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
+                                    p["f_number"], p["rx_start_s"], p["sampling_freq_hz"], p["sound_speed_m_s"],
+                                    p["modulation_freq_hz"], p["tukey_alpha"], p["interp_type"], accumulate=False,
+                                    layout=layout)
+
+    best, times = tune(run, acq.n_frames, reps)
+    key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox)
+    if record:
+        table().record(key, best, times, {"workload": acq.name, "storage": storage})
+    return key, best, times
+
+
+def _main():
+    import argparse
+
+    from . import phantoms
+
+    ap = argparse.ArgumentParser(description="Tune kernel layouts for the benchmark workloads and save the table")
+    ap.add_argument("--workloads", default="plane_wave,compound,doppler,volumetric")
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    dev = torch.device("cuda", 0)
+    for name in args.workloads.split(","):
+        acq = phantoms.make(name, device=dev)
+        storages = ("fp16",) if name == "volumetric" else ("c64", "fp16")
+        for st in storages:
+            key, best, times = tune_acquisition(acq, st, args.reps)
+            ranked = sorted(times.items(), key=lambda kv: kv[1])
+            print(f"{name:11s} {st:5s} {key}: best {hex(best)} {times[best]:.4f} ms; "
+                  + ", ".join(f"{hex(k)}={v:.4f}" for k, v in ranked[:6]), flush=True)
+        del acq
+        torch.cuda.empty_cache()
+    print("saved", table().save(args.out), flush=True)
+
+
+if __name__ == "__main__":
+    _main()

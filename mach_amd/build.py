@@ -1,6 +1,7 @@
 """Builds the native library ``mach_amd/_lib/libmach_hip.so`` with hipcc for gfx950.
 
-Run as ``python -m mach_amd.build``. The build always recompiles. Flags:
+Run as ``python mach_amd/build.py`` (a plain script: importing the package would first load the
+library being rebuilt). The build always recompiles. Flags:
   * ``-ffp-contract=off``: products and sums round separately, so the float64 geometry makes the same
     discrete decisions (aperture, bounds, tap index) as a float64 evaluation of the same formulas;
     the kernels spell out every fused multiply-add they want with ``fmaf``.
@@ -17,7 +18,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
-SOURCES = ["das_kernel.hip", "mach_capi.hip"]
+SOURCES = ["das_kernel.hip", "das_batch.hip", "mach_capi.hip"]
 ARCH = "gfx950"
 
 

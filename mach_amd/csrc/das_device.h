@@ -1,0 +1,283 @@
+// Device building blocks of the delay-and-sum kernels (shared by the plain and the batched kernel).
+//
+// Computes, for every voxel v and frame f (reference algorithm: src/mach/kernel.cu:523-682):
+//   out[v][f] (+)= sum_t sum_e  w(v,e) * interp(D[t][e][:][f], s(v,e,t)) * exp(+j 2 pi f_mod tau(v,e,t))
+// with the aperture test, Tukey weight, delay and interpolation bounds of kernel.cu:106-252, 333-349
+// and 444-483.
+//
+// Work decomposition (CDNA4, wave64):
+//   * A wave owns 64/LF voxels; the LF lanes of one voxel split its frames, FPL consecutive frames per
+//     lane, so channel-data rows ([channel][sample][frame], frames contiguous) are read with
+//     LF*FPL-frame coalesced vector loads.
+//   * Per (voxel, element, transmit) geometry is computed once, in float64, by one of the LF lanes
+//     ("setup"), and broadcast to the other lanes of the voxel (v_readlane for LF = 64, ds_bpermute
+//     otherwise). Setup produces the first tap index, the interpolation fraction and the complex
+//     coefficient w*exp(j*phase), with the phase reduced modulo one cycle in float64 before the
+//     float32 sincospi. The frame loop is pure float32 FMA work on registers.
+//   * Transmits are summed inside the kernel (coherent compounding) and each output element has a
+//     single owner lane: one read-modify-write of `out`, no atomics.
+//   * The float64 setup makes every discrete decision (aperture, tau < 0, w == 0, interpolation
+//     bounds, tap index) bit-identical to a float64 evaluation of the same formulas; the library is
+//     built with -ffp-contract=off so products and sums round separately.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "das_kernel.h"
+
+namespace mach {
+namespace dev {
+
+template <int N>
+using fvec = float __attribute__((ext_vector_type(N)));
+template <int N>
+using hvec = _Float16 __attribute__((ext_vector_type(N)));
+
+constexpr int kNearest = 0, kLinear = 1, kQuadratic = 2;
+
+struct PairSetup {
+    int i0;      // first tap sample index, -1 = no contribution
+    float frac;  // linear: lerp fraction in [0, 1]; quadratic: x = s - round(s) in [-0.5, 0.5]
+    float wc;    // w * cos(phase)
+    float ws;    // w * sin(phase)
+};
+
+// Per (voxel, element, transmit) geometry in float64.
+template <int INTERP>
+__device__ __forceinline__ PairSetup pair_setup(const DasParams &p, double xv, double yv, double zv,
+                                                double R2, double inv_R, double Tv, int64_t e) {
+    PairSetup r{-1, 0.f, 0.f, 0.f};
+    const double dx = (double)p.rx[3 * e + 0] - xv;
+    const double dy = (double)p.rx[3 * e + 1] - yv;
+    const double dz = (double)p.rx[3 * e + 2] - zv;
+    const double h2 = dx * dx + dy * dy;
+    if (h2 > R2) return r;  // outside the f-number aperture (kernel.cu:462)
+    const double tau = Tv + sqrt(h2 + dz * dz) * p.inv_c;
+    float w = 1.f;
+    if (p.apod) {  // Tukey half window (kernel.cu:333-349)
+        const double rr = (h2 == 0.0) ? 0.0 : sqrt(h2) * inv_R;
+        if (rr < 0.0 || rr > 1.0) {
+            w = 0.f;
+        } else if (!(rr <= 1.0 - p.alpha)) {
+            // 0.5 - 0.5 cos(2x) == sin(x)^2, accurate near the window edge
+            const float xh = (float)((1.0 - rr) * p.half_pi_over_alpha);
+            const float sn = sinf(xh);
+            w = sn * sn;
+        }
+    }
+    if (tau < 0.0 || w == 0.f) return r;  // kernel.cu:634
+    const double s = (tau - p.t0) * p.fs;
+    const double ns = (double)p.n_samples;
+    int i0;
+    float frac = 0.f;
+    if constexpr (INTERP == kNearest) {
+        if (s < -0.5 || s > ns - 0.5) return r;
+        const double k = rint(s);  // round half to even
+        i0 = (int)fmin(k, ns - 1.0);  // s in (n-1, n-0.5] would round to n: clamp to the last sample
+    } else if constexpr (INTERP == kLinear) {
+        if (s < 0.0 || s > ns - 1.0) return r;
+        const double fl = floor(s);
+        i0 = (int)fl;
+        frac = (float)(s - fl);
+        if (i0 == p.n_samples - 1 && p.n_samples >= 2) {  // s == n-1 exactly: taps (n-2, n-1), t = 1
+            i0 -= 1;
+            frac = 1.f;
+        }
+    } else {
+        if (s < 1.0 || s > ns - 2.0) return r;
+        const double c = rint(s);
+        i0 = (int)c - 1;
+        frac = (float)(s - c);
+    }
+    float wc = w, ws = 0.f;
+    if (p.rotate) {
+        const double cyc = p.fmod * tau;            // phase in cycles, float64
+        const float red = (float)(cyc - rint(cyc));  // reduced to [-0.5, 0.5] cycles, exact subtraction
+        float sn, cs;
+        sincospif(2.f * red, &sn, &cs);
+        wc = w * cs;
+        ws = w * sn;
+    }
+    r.i0 = i0;
+    r.frac = frac;
+    r.wc = wc;
+    r.ws = ws;
+    return r;
+}
+
+template <int LF>
+__device__ __forceinline__ int bcast(int x, int j, int lane) {
+    if constexpr (LF == 1) {
+        return x;
+    } else if constexpr (LF == 64) {
+        return __builtin_amdgcn_readlane(x, j);
+    } else {
+        return __builtin_amdgcn_ds_bpermute(((lane & ~(LF - 1)) + j) << 2, x);
+    }
+}
+
+// Loads FPL consecutive frames of one sample row. `off` is the element offset of the first frame.
+template <int ST, int FPL>
+__device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t off, float (&re)[FPL],
+                                         float (&im)[FPL]) {
+    if constexpr (ST == kStoreF32) {
+        const float *q = (const float *)base + off;
+        if constexpr (FPL == 1) {
+            re[0] = q[0];
+        } else {
+            const fvec<FPL> v = *(const fvec<FPL> *)q;
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) re[k] = v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) im[k] = 0.f;
+    } else if constexpr (ST == kStoreC64) {
+        const float *q = (const float *)base + 2 * off;
+        if constexpr (FPL == 1) {
+            const fvec<2> v = *(const fvec<2> *)q;
+            re[0] = v[0];
+            im[0] = v[1];
+        } else if constexpr (FPL == 2) {
+            const fvec<4> v = *(const fvec<4> *)q;
+            re[0] = v[0]; im[0] = v[1]; re[1] = v[2]; im[1] = v[3];
+        } else {
+            const fvec<4> a = *(const fvec<4> *)q;
+            const fvec<4> b = *(const fvec<4> *)(q + 4);
+            re[0] = a[0]; im[0] = a[1]; re[1] = a[2]; im[1] = a[3];
+            re[2] = b[0]; im[2] = b[1]; re[3] = b[2]; im[3] = b[3];
+        }
+    } else {
+        const _Float16 *q = (const _Float16 *)base + 2 * off;
+        const hvec<2 * FPL> v = *(const hvec<2 * FPL> *)q;
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            re[k] = (float)v[2 * k];
+            im[k] = (float)v[2 * k + 1];
+        }
+    }
+}
+
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, int64_t off, int64_t step,
+                                                float frac, float wc, float ws, float (&acc_re)[FPL],
+                                                float (&acc_im)[FPL]) {
+    constexpr bool kComplex = ST != kStoreF32;
+    float vr[FPL], vi[FPL];
+    if constexpr (INTERP == kNearest) {
+        load_row<ST, FPL>(data, off, vr, vi);
+    } else if constexpr (INTERP == kLinear) {
+        float ar[FPL], ai[FPL], br[FPL], bi[FPL];
+        load_row<ST, FPL>(data, off, ar, ai);
+        load_row<ST, FPL>(data, off + step, br, bi);
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {  // lerp a + t (b - a) (vendor/helper_math.h:1150-1152)
+            vr[k] = fmaf(frac, br[k] - ar[k], ar[k]);
+            if constexpr (kComplex) vi[k] = fmaf(frac, bi[k] - ai[k], ai[k]);
+        }
+    } else {
+        float ar[FPL], ai[FPL], br[FPL], bi[FPL], cr[FPL], ci[FPL];
+        load_row<ST, FPL>(data, off, ar, ai);
+        load_row<ST, FPL>(data, off + step, br, bi);
+        load_row<ST, FPL>(data, off + 2 * step, cr, ci);
+        const float x = frac;  // Lagrange weights about the centre tap (kernel.cu:233-240)
+        const float l0 = 0.5f * x * (x - 1.f);
+        const float l1 = (1.f - x) * (1.f + x);
+        const float l2 = 0.5f * x * (x + 1.f);
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            vr[k] = fmaf(l2, cr[k], fmaf(l1, br[k], l0 * ar[k]));
+            if constexpr (kComplex) vi[k] = fmaf(l2, ci[k], fmaf(l1, bi[k], l0 * ai[k]));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+        if constexpr (kComplex) {
+            acc_re[k] = fmaf(wc, vr[k], fmaf(-ws, vi[k], acc_re[k]));
+            acc_im[k] = fmaf(wc, vi[k], fmaf(ws, vr[k], acc_im[k]));
+        } else {
+            acc_re[k] = fmaf(wc, vr[k], acc_re[k]);
+        }
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id i runs on XCD i % 8). Remap so
+// that each XCD walks one contiguous range of logical blocks: neighbouring voxels (which read the same
+// channel-data rows) then share an L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = n / kXcd, rem = n % kXcd;
+    const uint32_t xcd = bid % kXcd, local = bid / kXcd;
+    return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
+// order) and frame chunk `by` (LF * FPL frames). Used by the plain kernel and the batched kernel.
+template <int ST, int INTERP, int LF, int FPL>
+__device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
+    constexpr int LV = 64 / LF;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int lv = lane / LF;
+    const int lf = lane % LF;
+    const int64_t vgroup = (int64_t)bx * 4 + wave;
+    const int64_t slot = vgroup * LV + lv;
+    const bool vok = slot < p.n_vox;
+    const int64_t v = (vok && p.order != nullptr) ? (int64_t)p.order[slot] : slot;
+    const int64_t f0 = (int64_t)by * (LF * FPL) + (int64_t)lf * FPL;
+    const bool fok = f0 < p.n_frames;
+
+    double xv = 0.0, yv = 0.0, zv = 1.0;
+    if (vok) {
+        xv = (double)p.vox[3 * v + 0];
+        yv = (double)p.vox[3 * v + 1];
+        zv = (double)p.vox[3 * v + 2];
+    }
+    const double R = zv / p.two_f_number;  // aperture radius (kernel.cu:583)
+    const double R2 = R * R;
+    const double inv_R = 1.0 / R;
+
+    float acc_re[FPL], acc_im[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) acc_re[k] = acc_im[k] = 0.f;
+
+    for (int64_t t = 0; t < p.n_tx; ++t) {
+        const double Tv = vok ? (double)p.arrivals[t * p.n_vox + v] : 0.0;
+        const int64_t tx_base = t * p.n_rx;
+        for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
+            const int64_t e = e0 + lf;
+            PairSetup s{-1, 0.f, 0.f, 0.f};
+            if (vok && e < p.n_rx) s = pair_setup<INTERP>(p, xv, yv, zv, R2, inv_R, Tv, e);
+            const int nj = (int)((p.n_rx - e0) < LF ? (p.n_rx - e0) : LF);
+            for (int j = 0; j < nj; ++j) {
+                const int i0 = bcast<LF>(s.i0, j, lane);
+                const float frac = __int_as_float(bcast<LF>(__float_as_int(s.frac), j, lane));
+                const float wc = __int_as_float(bcast<LF>(__float_as_int(s.wc), j, lane));
+                const float ws = __int_as_float(bcast<LF>(__float_as_int(s.ws), j, lane));
+                if (i0 >= 0 && fok) {
+                    const int64_t off = ((tx_base + e0 + j) * p.n_samples + i0) * p.n_frames + f0;
+                    accumulate_pair<ST, INTERP, FPL>(p.data, off, p.tap_step, frac, wc, ws, acc_re, acc_im);
+                }
+            }
+        }
+    }
+
+    if (vok && fok) {
+        const int64_t o = v * p.n_frames + f0;
+        if constexpr (ST == kStoreF32) {
+            float *q = (float *)p.out + o;
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) q[k] = p.accumulate ? q[k] + acc_re[k] : acc_re[k];
+        } else {
+            float *q = (float *)p.out + 2 * o;
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) {
+                q[2 * k + 0] = p.accumulate ? q[2 * k + 0] + acc_re[k] : acc_re[k];
+                q[2 * k + 1] = p.accumulate ? q[2 * k + 1] + acc_im[k] : acc_im[k];
+            }
+        }
+    }
+}
+
+}  // namespace dev
+}  // namespace mach

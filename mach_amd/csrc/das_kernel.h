@@ -18,6 +18,7 @@ struct DasParams {
     const float *vox;        // [n_vox][3]
     const float *arrivals;   // [n_tx][n_vox]
     void *out;               // [n_vox][n_frames]
+    const int32_t *order;    // optional [n_vox] processing order (nullptr = identity)
     int64_t n_tx, n_rx, n_samples, n_frames, n_vox;
     int64_t tap_step;        // elements between interpolation taps: n_frames (0 when n_samples == 1)
     double two_f_number;     // 2 * F
@@ -42,6 +43,9 @@ struct DasLayout {
 
 // Picks the layout for a problem (frame count and pointer alignment decide vector widths).
 DasLayout choose_layout(const DasParams &p, int store);
+// True if `l` is a compiled layout whose frames-per-lane divides n_frames and keeps vector loads and
+// stores aligned for these pointers.
+bool layout_valid(const DasParams &p, int store, DasLayout l);
 
 // Enqueues the delay-and-sum kernel. Returns hipSuccess or the launch error.
 hipError_t launch_das(const DasParams &p, int store, int interp, DasLayout layout, hipStream_t stream);

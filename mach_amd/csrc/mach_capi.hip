@@ -9,8 +9,15 @@
 #include <cstring>
 #include <string>
 
+#include <vector>
+
 #include "../../include/mach_hip.h"
+#include "das_batch.h"
 #include "das_kernel.h"
+
+struct mach_batch {
+    mach::BatchState *state;
+};
 
 namespace {
 
@@ -61,6 +68,7 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->vox = a->scan_coords_m;
     p->arrivals = a->tx_wave_arrivals_s;
     p->out = a->out;
+    p->order = a->voxel_order;
     p->n_tx = (int64_t)a->n_tx;
     p->n_rx = (int64_t)a->n_rx;
     p->n_samples = (int64_t)a->n_samples;
@@ -85,15 +93,79 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
 
 extern "C" {
 
+static int resolve_layout(const mach_beamform_args *a, const mach::DasParams &p, mach::DasLayout *out) {
+    if (a->layout == 0) {
+        *out = mach::choose_layout(p, a->data_type);
+        return MACH_OK;
+    }
+    const mach::DasLayout l{(a->layout >> 4) & 0xfff, a->layout & 0xf};
+    if (!mach::layout_valid(p, a->data_type, l))
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
+                                                   " is not valid for this request (lanes_per_voxel must be a power of"
+                                                   " two <= 64, frames_per_lane in {1, 2, 4} dividing n_frames and"
+                                                   " keeping vector accesses aligned)");
+    *out = l;
+    return MACH_OK;
+}
+
 int mach_beamform(const mach_beamform_args *args, void *stream) {
     g_last_error.clear();
     mach::DasParams p;
-    const int st = validate_and_build(args, &p);
+    int st = validate_and_build(args, &p);
     if (st != MACH_OK) return st;
     if (p.n_vox == 0 || p.n_frames == 0) return MACH_OK;
-    const mach::DasLayout layout = mach::choose_layout(p, args->data_type);
+    mach::DasLayout layout;
+    st = resolve_layout(args, p, &layout);
+    if (st != MACH_OK) return st;
     const hipError_t e = mach::launch_das(p, args->data_type, args->interp_type, layout, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "beamform kernel launch");
+    return MACH_OK;
+}
+
+int mach_batch_create(uint32_t max_requests, int32_t use_graph, mach_batch **batch) {
+    g_last_error.clear();
+    if (batch == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: batch is NULL");
+    *batch = nullptr;
+    if (max_requests == 0 || max_requests > (1u << 20))
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: max_requests must be in [1, 1048576]");
+    mach::BatchState *s = nullptr;
+    const hipError_t e = mach::batch_create(max_requests, use_graph != 0, &s);
+    if (e != hipSuccess) return hip_fail(e, "mach_batch_create");
+    *batch = new mach_batch{s};
+    return MACH_OK;
+}
+
+int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uint32_t n_requests, void *stream) {
+    g_last_error.clear();
+    if (batch == nullptr || batch->state == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: batch is NULL");
+    if (n_requests > mach::batch_capacity(batch->state))
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: " + std::to_string(n_requests) +
+                                                   " requests exceed the batch capacity of " +
+                                                   std::to_string(mach::batch_capacity(batch->state)));
+    if (n_requests > 0 && requests == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: requests is NULL");
+    std::vector<mach::DasParams> params(n_requests);
+    std::vector<int> stores(n_requests), interps(n_requests);
+    std::vector<mach::DasLayout> layouts(n_requests);
+    for (uint32_t i = 0; i < n_requests; ++i) {
+        int st = validate_and_build(&requests[i], &params[i]);
+        if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0)
+            st = resolve_layout(&requests[i], params[i], &layouts[i]);
+        if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
+        layouts[i] = mach::batch_layout(params[i], requests[i].data_type, layouts[i]);
+        stores[i] = requests[i].data_type;
+        interps[i] = requests[i].interp_type;
+    }
+    const hipError_t e = mach::batch_submit(batch->state, params.data(), stores.data(), interps.data(),
+                                            layouts.data(), n_requests, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mach_batch_submit");
+    return MACH_OK;
+}
+
+int mach_batch_destroy(mach_batch *batch) {
+    g_last_error.clear();
+    if (batch == nullptr) return MACH_OK;
+    mach::batch_destroy(batch->state);
+    delete batch;
     return MACH_OK;
 }
 

@@ -173,6 +173,48 @@ def _grid(nx, nz, x_span, z_span, ny=None, y_span=None):
     return probes.scan_grid(x, y, z).astype(np.float32), ((nx, nz) if ny is None else (nx, ny, nz))
 
 
+def _compound_angles(n_tx):
+    return np.deg2rad(np.linspace(-16.0, 16.0, n_tx))
+
+
+VOLUMETRIC_TILTS_DEG = [(0.0, 0.0), (8.0, 0.0), (-8.0, 0.0), (0.0, 8.0), (0.0, -8.0)]
+
+
+def _volumetric_sources():
+    """Virtual sources 10 mm behind the aperture centre, along the tilted axes."""
+    return np.array([-10e-3 * np.asarray(geometry.ultrasound_angles_to_cartesian(np.deg2rad(a), np.deg2rad(b)))
+                     for a, b in VOLUMETRIC_TILTS_DEG])
+
+
+def workload_grid(name: str, grid_scale: float = 1.0, n_tx: int | None = None):
+    """Scan grid [V, 3] float32, grid shape and transmit arrivals [T, V] float32 of a workload with
+    every grid axis scaled by ``grid_scale``."""
+    if name in ("plane_wave", "doppler"):
+        n = max(8, int(round(256 * grid_scale)))
+        scan, gshape = _grid(n, n, (-19e-3, 19e-3), (5e-3, 45e-3))
+        return scan, gshape, plane_wave_arrivals(scan, [0.0])
+    if name == "compound":
+        nx, nz = max(8, int(round(387 * grid_scale))), max(8, int(round(609 * grid_scale)))
+        scan, gshape = _grid(nx, nz, (-19e-3, 19e-3), (5e-3, 50e-3))
+        return scan, gshape, plane_wave_arrivals(scan, _compound_angles(75 if n_tx is None else n_tx))
+    if name == "volumetric":
+        n = max(4, int(round(128 * grid_scale)))
+        x = np.linspace(-10e-3, 10e-3, n)
+        z = np.linspace(5e-3, 45e-3, n)
+        scan = probes.scan_grid(x, x, z).astype(np.float32)
+        return scan, (n, n, n), diverging_wave_arrivals(scan, _volumetric_sources())
+    raise KeyError(name)
+
+
+def regrid(acq: "Acquisition", grid_scale: float) -> "Acquisition":
+    """Same acquisition (channel data, array, parameters) beamformed on a grid scaled by ``grid_scale``."""
+    scan, gshape, arrivals = workload_grid(acq.name, grid_scale, acq.n_tx)
+    dev = acq.channel_data.device
+    return dataclasses.replace(acq, scan_coords_m=torch.as_tensor(scan, device=dev).contiguous(),
+                               tx_wave_arrivals_s=torch.as_tensor(arrivals, device=dev).contiguous(),
+                               grid_shape=gshape)
+
+
 def _finish(name, data, rx, scan, arrivals, grid_shape, params, seed, device):
     dev = torch.device(device)
     return Acquisition(
@@ -205,8 +247,7 @@ def plane_wave(device="cpu", seed=0, scale=1.0) -> Acquisition:
     data = synthesize_echoes(pts[None], amps, rx, lambda p, t: _tx_delay_plane(p, 0.0, C_M_S), 1, n_s,
                              device=device)
     data = _add_noise(data, rng)
-    scan, gshape = _grid(n_grid, n_grid, (-19e-3, 19e-3), (5e-3, 45e-3))
-    arrivals = plane_wave_arrivals(scan, [0.0])
+    scan, gshape, arrivals = workload_grid("plane_wave", n_grid / 256)
     return _finish("plane_wave", data, rx, scan, arrivals, gshape, _base_params(1.7, 1), seed, device)
 
 
@@ -227,7 +268,7 @@ def compound(device="cpu", seed=1, scale=1.0, n_angles=75) -> Acquisition:
             pts.append((x, 0.0, z))
     pts = np.asarray(pts)
     amps = (rng.standard_normal(n_sc) + 1j * rng.standard_normal(n_sc)) / math.sqrt(2)
-    angles = np.deg2rad(np.linspace(-16.0, 16.0, n_tx))
+    angles = _compound_angles(n_tx)
     data = synthesize_echoes(pts[None], amps, rx, lambda p, t: _tx_delay_plane(p, angles[t], C_M_S), n_tx, n_s,
                              device=device)
     data = _add_noise(data, rng)
@@ -259,8 +300,7 @@ def doppler(device="cpu", seed=2, scale=1.0, n_frames=256) -> Acquisition:
     data = synthesize_echoes(disk, amps_d, rx, txd, 1, n_s, device=device)
     data = data + synthesize_echoes(clutter[None], amps_c, rx, txd, 1, n_s, device=device)  # broadcast over frames
     data = _add_noise(data, rng)
-    scan, gshape = _grid(n_grid, n_grid, (-19e-3, 19e-3), (5e-3, 45e-3))
-    arrivals = plane_wave_arrivals(scan, [0.0])
+    scan, gshape, arrivals = workload_grid("doppler", n_grid / 256)
     return _finish("doppler", data, rx, scan, arrivals, gshape, _base_params(1.7, 2), seed, device)
 
 
@@ -273,9 +313,8 @@ def volumetric(device="cpu", seed=3, scale=1.0, n_frames=8) -> Acquisition:
     n_grid = max(4, int(round(128 * scale)))
     n_sc = max(8, int(round(5000 * scale)))
     rx = probes.matrix_probe_positions(n_side, n_side, PITCH_M)
-    tilts = [(0.0, 0.0), (8.0, 0.0), (-8.0, 0.0), (0.0, 8.0), (0.0, -8.0)]
-    sources = np.array([-10e-3 * np.asarray(geometry.ultrasound_angles_to_cartesian(np.deg2rad(a), np.deg2rad(b)))
-                        for a, b in tilts])
+    tilts = VOLUMETRIC_TILTS_DEG
+    sources = _volumetric_sources()
     pts = np.stack([rng.uniform(-10e-3, 10e-3, n_sc), rng.uniform(-10e-3, 10e-3, n_sc),
                     rng.uniform(5e-3, 45e-3, n_sc)], -1)
     amps = (rng.standard_normal(n_sc) + 1j * rng.standard_normal(n_sc)) / math.sqrt(2)
@@ -283,10 +322,7 @@ def volumetric(device="cpu", seed=3, scale=1.0, n_frames=8) -> Acquisition:
                              len(tilts), n_s, device=device)
     data = data.expand(-1, -1, -1, n_frames).contiguous()
     data = _add_noise(data, rng)
-    x = np.linspace(-10e-3, 10e-3, n_grid)
-    z = np.linspace(5e-3, 45e-3, n_grid)
-    scan = probes.scan_grid(x, x, z).astype(np.float32)
-    arrivals = diverging_wave_arrivals(scan, sources)
+    scan, _, arrivals = workload_grid("volumetric", n_grid / 128)
     return _finish("volumetric", data, rx, scan, arrivals, (n_grid, n_grid, n_grid), _base_params(1.5, 1), seed, device)
 
 

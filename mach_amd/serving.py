@@ -1,0 +1,355 @@
+"""Continuous-batching beamform server over device-resident ring buffers.
+
+Requests (any workload shape, transmit count, frame count, storage type) arrive over time. The
+server:
+
+* keeps precomputed geometry on the device, per scan geometry: element and voxel coordinates,
+  transmit arrivals and the depth-major voxel order (``GeometryCache``);
+* keeps channel data and outputs in device ring buffers (``DeviceRing``). A request's channel data
+  lands in its ring slot when it arrives, the way an acquisition front end DMAs into device memory;
+* admits arrived requests in arrival order while their channel-data and output bytes fit the HBM
+  budget (the rings' capacity). Later arrivals wait until completed work frees space;
+* packs every admitted, not-yet-running request into one persistent-kernel launch (``BatchRunner``,
+  graph replay). Up to two batches are in flight, on two streams, so requests that arrive while a
+  batch runs are admitted and start while the previous batch drains;
+* reports per-request latency (arrival -> result ready on the device), p50/p99, and throughput in
+  points/s over the makespan.
+
+``serve_plain`` runs the same arrival stream one request at a time through the plain kernel (one
+launch and one synchronisation per request): the baseline the batched server is compared with.
+Each request's output is identical in both (same kernel code and per-voxel summation order), which
+``tests/test_gpu_serving.py`` checks.
+"""
+
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import batch as _batch
+from . import phantoms
+from ._hip_impl import voxel_order
+
+GIB = 1 << 30
+
+
+@dataclasses.dataclass
+class Geometry:
+    rx_coords_m: torch.Tensor
+    scan_coords_m: torch.Tensor
+    tx_wave_arrivals_s: torch.Tensor
+    voxel_order: torch.Tensor | None
+    params: dict
+
+
+@dataclasses.dataclass
+class ServeRequest:
+    rid: int
+    arrival_s: float
+    geometry: Geometry
+    source: torch.Tensor  # resident template channel data [T, E, S, F] the front end "DMAs" from
+    kind: str  # workload name
+    points: int
+    # filled by the server
+    admit_s: float = float("nan")
+    done_s: float = float("nan")
+    out: torch.Tensor | None = None
+    data_slot: tuple | None = None
+    out_slot: tuple | None = None
+
+    @property
+    def data_bytes(self) -> int:
+        return self.source.numel() * self.source.element_size()
+
+    @property
+    def out_bytes(self) -> int:
+        return self.geometry.scan_coords_m.shape[0] * self.source.shape[3] * 8
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+
+class DeviceRing:
+    """First-in first-out byte ring on the device. Allocations are 256-byte aligned and contiguous
+    (an allocation that would cross the end starts again at 0); frees happen in allocation order,
+    which is the order the server completes requests in."""
+
+    ALIGN = 256
+
+    def __init__(self, capacity_bytes: int, device):
+        self.capacity = int(capacity_bytes) // self.ALIGN * self.ALIGN
+        self.buf = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.head = 0
+        self.live = collections.deque()  # (start, size) in allocation order
+        self._undo = None
+
+    @property
+    def used(self) -> int:
+        return sum(n for _, n in self.live)
+
+    def alloc(self, nbytes: int):
+        """Reserve ``nbytes``; returns (start, size) or None if it does not fit now."""
+        n = (int(nbytes) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        if n > self.capacity:
+            raise ValueError(f"request of {nbytes} bytes exceeds the ring capacity {self.capacity}")
+        if not self.live:
+            start = 0
+        else:
+            tail = self.live[0][0]
+            if self.head > tail:  # live bytes occupy [tail, head)
+                if self.head + n <= self.capacity:
+                    start = self.head
+                elif n <= tail:
+                    start = 0
+                else:
+                    return None
+            elif self.head + n <= tail:  # wrapped: the free gap is [head, tail)
+                start = self.head
+            else:
+                return None
+        self._undo = self.head
+        self.live.append((start, n))
+        self.head = start + n
+        return (start, n)
+
+    def undo_last(self):
+        """Release the most recent allocation (admission of a request that did not fully fit)."""
+        self.live.pop()
+        self.head = self._undo if self.live else 0
+
+    def free_oldest(self):
+        self.live.popleft()
+        if not self.live:
+            self.head = 0
+
+    def view(self, slot, dtype, shape):
+        start, _ = slot
+        nbytes = int(np.prod(shape)) * torch.empty((), dtype=dtype).element_size()
+        return self.buf[start:start + nbytes].view(dtype).view(shape)
+
+
+class GeometryCache:
+    """Device-resident geometry per workload template and grid scale."""
+
+    def __init__(self, device):
+        self.device = device
+        self._cache = {}
+
+    def get(self, acq: phantoms.Acquisition, key) -> Geometry:
+        g = self._cache.get(key)
+        if g is None:
+            g = Geometry(acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s,
+                         voxel_order(acq.scan_coords_m), dict(acq.params))
+            self._cache[key] = g
+        return g
+
+
+def _frames_tile(data: torch.Tensor, n_frames: int) -> torch.Tensor:
+    """[T, E, S, F0] -> [T, E, S, n_frames] by repeating frames (contiguous)."""
+    reps = (n_frames + data.shape[3] - 1) // data.shape[3]
+    return data.repeat(1, 1, 1, reps)[..., :n_frames].contiguous()
+
+
+def make_stream(device, n_requests=120, qps=200.0, seed=4, scales=(0.5, 0.625, 0.75, 0.875, 1.0),
+                frames=(1, 2, 4, 8, 16, 32, 64, 128, 256), workloads=("plane_wave", "compound", "doppler", "volumetric"),
+                template_scale=1.0):
+    """Seeded mixed request stream: Poisson arrivals at ``qps``. Each request draws a workload
+    (the four benchmark configurations), a grid scale (every grid axis x0.5-1.0) and, for the
+    single-plane ensembles (plane_wave, doppler), a frame count 1-256 (powers of two); compound keeps
+    its single frame and volumetric its 8 FP16 frames. One channel-data template per workload is
+    synthesized (at ``template_scale``) and stays resident; geometry is built once per grid scale."""
+    rng = np.random.default_rng(seed)
+    cache = GeometryCache(device)
+    templates, grids, sources = {}, {}, {}
+    reqs = []
+    t = 0.0
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / qps)
+        name = workloads[rng.integers(len(workloads))]
+        scale = float(scales[rng.integers(len(scales))])
+        if name in ("plane_wave", "doppler"):
+            nf = int(frames[rng.integers(len(frames))])
+        elif name == "compound":
+            nf = 1
+        else:
+            nf = 8
+        if name not in templates:
+            kw = {"scale": template_scale}
+            if name == "doppler":
+                kw["n_frames"] = 1
+            templates[name] = phantoms.make(name, device=device, **kw)
+        key = (name, scale)
+        if key not in grids:
+            grids[key] = phantoms.regrid(templates[name], scale * template_scale)
+        acq = grids[key]
+        geo = cache.get(acq, key)
+        skey = (name, nf)
+        if skey not in sources:
+            if name in ("plane_wave", "doppler"):
+                sources[skey] = _frames_tile(acq.channel_data, nf)
+            elif name == "volumetric":
+                sources[skey] = acq.fp16_channel_data()
+            else:
+                sources[skey] = acq.channel_data
+        src = sources[skey]
+        pts = geo.scan_coords_m.shape[0] * acq.n_rx * nf * acq.n_tx
+        reqs.append(ServeRequest(rid, t, geo, src, name, pts))
+    torch.cuda.synchronize()
+    return reqs
+
+
+def _out_view(req: ServeRequest, ring: DeviceRing, slot):
+    return ring.view(slot, torch.complex64, (req.geometry.scan_coords_m.shape[0], req.source.shape[3]))
+
+
+def _data_view(req: ServeRequest, ring: DeviceRing, slot):
+    return ring.view(slot, req.source.dtype, tuple(req.source.shape))
+
+
+def _to_request(req: ServeRequest, data: torch.Tensor, out: torch.Tensor) -> _batch.Request:
+    g = req.geometry
+    p = g.params
+    return _batch.Request(data, g.rx_coords_m, g.scan_coords_m, g.tx_wave_arrivals_s, out,
+                          f_number=p["f_number"], rx_start_s=p["rx_start_s"], sampling_freq_hz=p["sampling_freq_hz"],
+                          sound_speed_m_s=p["sound_speed_m_s"], modulation_freq_hz=p["modulation_freq_hz"],
+                          tukey_alpha=p["tukey_alpha"], interp_type=p["interp_type"], accumulate=False,
+                          voxel_order=g.voxel_order)
+
+
+@dataclasses.dataclass
+class ServeReport:
+    mode: str
+    n_requests: int
+    makespan_s: float
+    points: int
+    p50_ms: float
+    p99_ms: float
+    mean_ms: float
+    max_batch: int = 1
+    n_batches: int = 0
+
+    @property
+    def gpts_per_s(self) -> float:
+        return self.points / self.makespan_s / 1e9
+
+    def as_dict(self):
+        d = dataclasses.asdict(self)
+        d["gpts_per_s"] = self.gpts_per_s
+        return d
+
+
+def _report(mode, reqs, t_first, max_batch=1, n_batches=0) -> ServeReport:
+    lat = np.array([r.latency_s for r in reqs]) * 1e3
+    end = max(r.done_s for r in reqs)
+    return ServeReport(mode, len(reqs), end - t_first, sum(r.points for r in reqs),
+                       float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), float(lat.mean()),
+                       max_batch, n_batches)
+
+
+def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False):
+    """Continuous batching over the persistent kernel. Returns a ServeReport; with ``keep_outputs`` each
+    request's ``out`` is a private copy of its result."""
+    free, _total = torch.cuda.mem_get_info(device)
+    budget = min(int(budget_bytes), int(free * 0.8))
+    data_ring = DeviceRing(budget * 3 // 4, device)
+    out_ring = DeviceRing(budget - budget * 3 // 4, device)
+    copy_stream = torch.cuda.Stream(device)
+    compute = [torch.cuda.Stream(device), torch.cuda.Stream(device)]
+    runner = _batch.BatchRunner(max_requests=max_batch, use_graph=use_graph)
+    pending = collections.deque(sorted(reqs, key=lambda r: r.arrival_s))
+    waiting = collections.deque()
+    admitted = []  # (req, copy_event)
+    inflight = collections.deque()  # (reqs, event)
+    n_done, n_batches, biggest = 0, 0, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while n_done < len(reqs):
+        now = time.perf_counter() - t0
+        while pending and pending[0].arrival_s <= now:
+            waiting.append(pending.popleft())
+        # admission by HBM budget, in arrival order
+        while waiting:
+            r = waiting[0]
+            ds = data_ring.alloc(r.data_bytes)
+            if ds is None:
+                break
+            os_ = out_ring.alloc(r.out_bytes)
+            if os_ is None:
+                data_ring.undo_last()
+                break
+            waiting.popleft()
+            r.data_slot, r.out_slot, r.admit_s = ds, os_, now
+            with torch.cuda.stream(copy_stream):  # the front end's DMA into the ring slot
+                _data_view(r, data_ring, ds).copy_(r.source, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(copy_stream)
+            admitted.append((r, ev))
+        # completions (batches finish in order)
+        while inflight and inflight[0][1].query():
+            done_reqs, _ = inflight.popleft()
+            t_done = time.perf_counter() - t0
+            for r in done_reqs:
+                r.done_s = t_done
+                if keep_outputs:  # private copy, complete before the ring slot can be reused
+                    r.out = r.out.clone()
+                    torch.cuda.current_stream(device).synchronize()
+                data_ring.free_oldest()
+                out_ring.free_oldest()
+            n_done += len(done_reqs)
+        # launch: everything admitted goes into the next batch
+        if admitted and len(inflight) < 2:
+            take = admitted[:max_batch]
+            del admitted[:max_batch]
+            stream = compute[n_batches % 2]
+            for _r, ev in take:
+                stream.wait_event(ev)
+            breqs = []
+            for r, _ev in take:
+                r.out = _out_view(r, out_ring, r.out_slot)
+                breqs.append(_to_request(r, _data_view(r, data_ring, r.data_slot), r.out))
+            runner.submit(breqs, stream=stream)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            inflight.append(([r for r, _ in take], ev))
+            n_batches += 1
+            biggest = max(biggest, len(take))
+        elif not admitted and not inflight and pending and not waiting:
+            # idle until the next arrival
+            dt = pending[0].arrival_s - (time.perf_counter() - t0)
+            if dt > 2e-4:
+                time.sleep(dt - 1e-4)
+    torch.cuda.synchronize()
+    runner.close()
+    rep = _report("batched", reqs, reqs[0].arrival_s if reqs else 0.0, biggest, n_batches)
+    del data_ring, out_ring
+    return rep
+
+
+def serve_plain(reqs, device, keep_outputs=False):
+    """Baseline: each request, in arrival order, is copied into its own device buffer and beamformed
+    by one plain-kernel launch followed by a synchronisation."""
+    ordered = sorted(reqs, key=lambda r: r.arrival_s)
+    stream = torch.cuda.current_stream(device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in ordered:
+        dt = r.arrival_s - (time.perf_counter() - t0)
+        if dt > 2e-4:
+            time.sleep(dt - 1e-4)
+        while time.perf_counter() - t0 < r.arrival_s:
+            pass
+        r.admit_s = time.perf_counter() - t0
+        data = torch.empty_like(r.source)
+        data.copy_(r.source, non_blocking=True)
+        out = torch.empty((r.geometry.scan_coords_m.shape[0], r.source.shape[3]), dtype=torch.complex64,
+                          device=device)
+        _batch.run_plain(_to_request(r, data, out), stream=stream)
+        stream.synchronize()
+        r.done_s = time.perf_counter() - t0
+        r.out = out if keep_outputs else None
+    return _report("plain", ordered, ordered[0].arrival_s if ordered else 0.0)

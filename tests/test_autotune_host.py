@@ -1,0 +1,44 @@
+"""Autotuner host logic (no GPU): candidate layouts, encoding, shape keys and the JSON table."""
+
+import json
+
+from mach_amd import autotune
+
+
+def test_candidates_divide_frames_and_encode():
+    for frames in (1, 2, 3, 8, 12, 33, 64, 256):
+        cands = autotune.candidates(frames)
+        assert cands, frames
+        for lay in cands:
+            lf, k = autotune.decode(lay)
+            assert lf in autotune.LANES and k in autotune.FRAMES_PER_LANE
+            assert frames % k == 0
+            assert autotune.encode(lf, k) == lay
+    assert 0x404 in autotune.candidates(256)  # 64 lanes x 4 frames
+
+
+def test_shape_key_buckets_by_octave():
+    a = autotune.shape_key(1, 2, 1, 128, 256, 65536)
+    b = autotune.shape_key(1, 2, 1, 100, 256, 40000)
+    c = autotune.shape_key(1, 2, 1, 128, 128, 65536)
+    assert a == b and a != c
+
+
+def test_table_roundtrip_and_alignment_guard(tmp_path):
+    path = str(tmp_path / "t.json")
+    t = autotune.TuningTable(path)
+    key = autotune.shape_key(1, 1, 1, 128, 8, 4096)
+    t.record(key, autotune.encode(2, 4), {0x24: 0.5, 0x11: 0.7}, {"workload": "x"})
+    t.save()
+    with open(path) as fh:
+        data = json.load(fh)
+    assert data["entries"][key]["layout"] == 0x24
+    t2 = autotune.TuningTable(path)
+    assert t2.lookup(key) == 0x24 and t2.lookup("missing") == 0
+    old = autotune._TABLE
+    try:
+        autotune._TABLE = t2
+        assert autotune.layout_for(1, 1, 1, 128, 8, 4096, data_ptr=256, out_ptr=256) == 0x24
+        assert autotune.layout_for(1, 1, 1, 128, 8, 4096, data_ptr=8, out_ptr=256) == 0  # misaligned for 4 frames
+    finally:
+        autotune._TABLE = old

@@ -98,3 +98,27 @@ def test_device_query_reports_status():
         assert buf.value.decode().startswith("gfx")
     else:
         assert "no HIP device" in _lib.last_error()
+
+
+def test_batch_entry_points_report_errors_without_work():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.mach_batch_create(0, 1, ctypes.byref(h)) == 1
+    assert "max_requests" in _lib.last_error()
+    assert lib.mach_batch_submit(None, None, 0, None) == 1
+    assert lib.mach_batch_destroy(None) == 0
+    st = lib.mach_batch_create(4, 1, ctypes.byref(h))
+    if st == 0:  # a GPU is present
+        assert lib.mach_batch_submit(h, None, 0, None) == 0
+        assert lib.mach_batch_destroy(h) == 0
+    else:
+        assert st == 3 and "HIP error" in _lib.last_error()
+
+
+def test_layout_hint_is_validated():
+    lib = _lib.load()
+    a = _args(layout=(3 << 4) | 1)  # 3 lanes per voxel is not a power of two
+    a.out = 256  # non-NULL placeholder; validation fails before any device access
+    a.channel_data = a.rx_coords_m = a.scan_coords_m = a.tx_wave_arrivals_s = 256
+    assert lib.mach_beamform(ctypes.byref(a), None) == 1
+    assert "layout" in _lib.last_error()

@@ -262,3 +262,24 @@ def test_workloads_fp16_storage(device, name, scale, sample):
     db = bmode_db_diff(np.abs(y16).mean(-1), np.abs(ref32).mean(-1))
     log(f"  {name} fp16: rel_l2 same-data={e_same:.2e} end-to-end={e_e2e:.2e} B-mode dB diff={db:.3f}")
     assert e_same <= TOL_FP16_SAME_DATA and e_e2e <= TOL_FP16_END_TO_END
+
+
+@pytest.mark.parametrize("frames", [1, 8, 12, 256])
+def test_every_layout_gives_identical_results(device, frames):
+    """The autotuner's layouts differ only in work assignment: outputs must be bitwise identical."""
+    from mach_amd import _hip_impl, autotune
+
+    rng = np.random.default_rng(frames)
+    D, rx, vox, arr = random_case(rng, T=2, E=9, S=48, F=frames, V=150)
+    d, r, v, a = to_dev(D, rx, vox, arr, device=device)
+    outs = {}
+    for lay in autotune.candidates(frames):
+        out = torch.zeros((150, frames), dtype=torch.complex64, device=device)
+        _hip_impl.beamform_compound(d, r, v, a, out, 0.8, -1e-7, 20e6, 1540.0, 5e6, 0.5, 2, layout=lay)
+        outs[lay] = out
+    torch.cuda.synchronize()
+    ref = das64(D, rx, vox, arr, **KW, modulation_freq_hz=5e6, tukey_alpha=0.5, interp_type=2)
+    first = next(iter(outs.values()))
+    assert rel_l2(first.cpu().numpy(), ref) <= TOL_FP32
+    for lay, o in outs.items():
+        assert torch.equal(o, first), hex(lay)
