@@ -18,6 +18,7 @@ from __future__ import annotations
 import collections
 import ctypes
 import enum
+import os
 import warnings
 import weakref
 
@@ -142,9 +143,10 @@ def voxel_order(scan: torch.Tensor):
     same few channel-data rows per element, so those rows stay in L2. Any permutation gives the same
     results (each voxel is computed independently, with a fixed summation order); the order only
     affects speed. Cached per scan-coordinate tensor (identity and in-place version).
+    ``MACH_AMD_VOXEL_ORDER=0`` disables it (caller order).
     """
     n = scan.shape[0]
-    if n < _ORDER_MIN_VOXELS or not scan.is_cuda:
+    if n < _ORDER_MIN_VOXELS or not scan.is_cuda or os.environ.get("MACH_AMD_VOXEL_ORDER", "1") == "0":
         return None
     key = (id(scan), scan.data_ptr(), n)
     hit = _ORDER_CACHE.get(key)

@@ -21,7 +21,7 @@ import threading
 
 import torch
 
-LANES = (1, 2, 4, 8, 16, 32, 64)
+LANES = (1, 2, 4, 8, 16, 32, 64)  # 0 in a layout code selects the LDS-staged band kernel
 FRAMES_PER_LANE = (1, 2, 4)
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "gfx950.json")
 
@@ -43,15 +43,20 @@ def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox
     return f"d{kind}:i{int(interp)}:t{_bucket(n_tx)}:e{_bucket(n_rx)}:f{n_frames}:v{_bucket(n_vox)}"
 
 
-def candidates(n_frames: int):
+def candidates(n_frames: int, band: bool = True):
     """Layouts worth timing for ``n_frames``: frames-per-lane divides the frame count and a voxel's
-    lanes cover no more than twice the frames they have to share."""
+    lanes cover no more than twice the frames they have to share (or 8 lanes: extra lanes still split
+    the per-pair geometry of setup-bound, few-frame problems). With ``band``, the LDS-staged band
+    kernel (lanes_per_voxel code 0) is included where its 64-frame rows divide the frame count."""
     out = []
+    if band:
+        out += [encode(0, k) for k in FRAMES_PER_LANE if n_frames % (64 * k) == 0]
     for k in FRAMES_PER_LANE:
         if n_frames % k:
             continue
         for lf in LANES:
-            if lf * k <= max(2 * n_frames, 1) and (lf * k >= min(n_frames, 64) // 4 or lf == 1):
+            # lanes beyond the frames only help setup-bound problems (few frames): allow up to 8
+            if lf * k <= max(2 * n_frames, 8) and (lf * k >= min(n_frames, 64) // 4 or lf == 1):
                 out.append(encode(lf, k))
     return out
 

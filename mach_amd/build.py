@@ -18,7 +18,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
-SOURCES = ["das_kernel.hip", "das_batch.hip", "mach_capi.hip"]
+SOURCES = ["das_kernel.hip", "das_band.hip", "das_batch.hip", "mach_capi.hip"]
 ARCH = "gfx950"
 
 

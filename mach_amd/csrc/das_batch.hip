@@ -48,17 +48,24 @@ struct BatchItem {
     uint32_t pad[2];
 };
 
+template <int ST, int INTERP, int TC>
+__device__ __forceinline__ void run_layout_tc(const BatchItem &it, uint32_t bx, uint32_t by) {
+    switch (it.layout) {
+        case 0: das_block<ST, INTERP, 64, 4, TC>(it.p, bx, by); break;
+        case 1: das_block<ST, INTERP, 64, 1, TC>(it.p, bx, by); break;
+        case 2: das_block<ST, INTERP, 16, 1, TC>(it.p, bx, by); break;
+        case 3: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by); break;
+        case 4: das_block<ST, INTERP, 4, 1, TC>(it.p, bx, by); break;
+        case 5: das_block<ST, INTERP, 1, 4, TC>(it.p, bx, by); break;
+        default: das_block<ST, INTERP, 1, 1, TC>(it.p, bx, by); break;
+    }
+}
+
+// Same transmit chunking as the plain kernel, so a request gives identical results in both.
 template <int ST, int INTERP>
 __device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by) {
-    switch (it.layout) {
-        case 0: das_block<ST, INTERP, 64, 4>(it.p, bx, by); break;
-        case 1: das_block<ST, INTERP, 64, 1>(it.p, bx, by); break;
-        case 2: das_block<ST, INTERP, 16, 1>(it.p, bx, by); break;
-        case 3: das_block<ST, INTERP, 4, 2>(it.p, bx, by); break;
-        case 4: das_block<ST, INTERP, 4, 1>(it.p, bx, by); break;
-        case 5: das_block<ST, INTERP, 1, 4>(it.p, bx, by); break;
-        default: das_block<ST, INTERP, 1, 1>(it.p, bx, by); break;
-    }
+    if (it.p.n_tx > 1) run_layout_tc<ST, INTERP, kTxChunk>(it, bx, by);
+    else run_layout_tc<ST, INTERP, 1>(it, bx, by);
 }
 
 template <int ST>

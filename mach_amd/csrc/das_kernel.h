@@ -10,6 +10,9 @@ namespace mach {
 
 enum StoreKind : int { kStoreF32 = 0, kStoreC64 = 1, kStoreC32H = 2 };
 
+// Transmits whose pair geometry is computed together (compounding / multi-transmit requests).
+constexpr int kTxChunk = 8;
+
 // Everything a launch needs, already validated and converted. Geometry scalars are float64 values
 // of the float32 API scalars.
 struct DasParams {

@@ -11,10 +11,10 @@ namespace {
 
 using namespace dev;
 
-template <int ST, int INTERP, int LF, int FPL>
+template <int ST, int INTERP, int LF, int FPL, int TC>
 __global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    das_block<ST, INTERP, LF, FPL>(p, lb % gridDim.x, lb / gridDim.x);
+    das_block<ST, INTERP, LF, FPL, TC>(p, lb % gridDim.x, lb / gridDim.x);
 }
 
 template <int ST, int INTERP, int LF, int FPL>
@@ -24,8 +24,12 @@ hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     const int64_t blocks_x = (n_vgroups + 3) / 4;
     const int64_t blocks_y = (p.n_frames + LF * FPL - 1) / (LF * FPL);
     if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL>), dim3((unsigned)blocks_x, (unsigned)blocks_y, 1),
-                       dim3(256, 1, 1), 0, stream, p);
+    const dim3 grid((unsigned)blocks_x, (unsigned)blocks_y, 1);
+    if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk>), grid, dim3(256, 1, 1), 0, stream, p);
+    } else {
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, 1>), grid, dim3(256, 1, 1), 0, stream, p);
+    }
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/mach_hip.h"
+#include "das_band.h"
 #include "das_batch.h"
 #include "das_kernel.h"
 
@@ -93,18 +94,41 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
 
 extern "C" {
 
-static int resolve_layout(const mach_beamform_args *a, const mach::DasParams &p, mach::DasLayout *out) {
-    if (a->layout == 0) {
-        *out = mach::choose_layout(p, a->data_type);
+// Kernel choice for one request: the LDS-staged band kernel (frame ensembles with a depth-major voxel
+// order) or the general kernel with a frame-lane layout. `layout` 0 chooses the general kernel's
+// heuristic layout; a value with lanes_per_voxel == 0 requests the band kernel with that many frames
+// per lane (the autotuner records it where it wins).
+struct Plan {
+    bool band;
+    int band_fpl;
+    mach::DasLayout layout;
+};
+
+static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, Plan *out) {
+    out->band = false;
+    out->band_fpl = 0;
+    if (a->layout == 0) {  // the band kernel is chosen only through a (tuned) layout
+        out->layout = mach::choose_layout(p, a->data_type);
         return MACH_OK;
     }
     const mach::DasLayout l{(a->layout >> 4) & 0xfff, a->layout & 0xf};
+    if (l.lanes_per_voxel == 0) {
+        const int k = mach::band_frames_per_lane(p, a->data_type);
+        if (k == 0 || (l.frames_per_lane != 1 && l.frames_per_lane != 2 && l.frames_per_lane != 4) ||
+            p.n_frames % (64 * l.frames_per_lane) != 0 || l.frames_per_lane > k)
+            return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
+                                                       " (band kernel) does not apply to this request");
+        out->band = true;
+        out->band_fpl = l.frames_per_lane;
+        out->layout = mach::choose_layout(p, a->data_type);
+        return MACH_OK;
+    }
     if (!mach::layout_valid(p, a->data_type, l))
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
                                                    " is not valid for this request (lanes_per_voxel must be a power of"
                                                    " two <= 64, frames_per_lane in {1, 2, 4} dividing n_frames and"
                                                    " keeping vector accesses aligned)");
-    *out = l;
+    out->layout = l;
     return MACH_OK;
 }
 
@@ -114,10 +138,12 @@ int mach_beamform(const mach_beamform_args *args, void *stream) {
     int st = validate_and_build(args, &p);
     if (st != MACH_OK) return st;
     if (p.n_vox == 0 || p.n_frames == 0) return MACH_OK;
-    mach::DasLayout layout;
-    st = resolve_layout(args, p, &layout);
+    Plan plan;
+    st = resolve_plan(args, p, &plan);
     if (st != MACH_OK) return st;
-    const hipError_t e = mach::launch_das(p, args->data_type, args->interp_type, layout, (hipStream_t)stream);
+    const hipError_t e =
+        plan.band ? mach::launch_das_band(p, args->data_type, args->interp_type, plan.band_fpl, (hipStream_t)stream)
+                  : mach::launch_das(p, args->data_type, args->interp_type, plan.layout, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "beamform kernel launch");
     return MACH_OK;
 }
@@ -148,9 +174,10 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
     std::vector<mach::DasLayout> layouts(n_requests);
     for (uint32_t i = 0; i < n_requests; ++i) {
         int st = validate_and_build(&requests[i], &params[i]);
-        if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0)
-            st = resolve_layout(&requests[i], params[i], &layouts[i]);
+        Plan plan{false, 0, mach::DasLayout{1, 1}};
+        if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0) st = resolve_plan(&requests[i], params[i], &plan);
         if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
+        layouts[i] = plan.layout;
         layouts[i] = mach::batch_layout(params[i], requests[i].data_type, layouts[i]);
         stores[i] = requests[i].data_type;
         interps[i] = requests[i].interp_type;

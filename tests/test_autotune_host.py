@@ -11,10 +11,11 @@ def test_candidates_divide_frames_and_encode():
         assert cands, frames
         for lay in cands:
             lf, k = autotune.decode(lay)
-            assert lf in autotune.LANES and k in autotune.FRAMES_PER_LANE
-            assert frames % k == 0
+            assert (lf in autotune.LANES or lf == 0) and k in autotune.FRAMES_PER_LANE
+            assert frames % k == 0 and (lf != 0 or frames % (64 * k) == 0)
             assert autotune.encode(lf, k) == lay
     assert 0x404 in autotune.candidates(256)  # 64 lanes x 4 frames
+    assert 0x2 in autotune.candidates(256) and 0x2 not in autotune.candidates(8)  # band kernel
 
 
 def test_shape_key_buckets_by_octave():

@@ -273,7 +273,7 @@ def test_every_layout_gives_identical_results(device, frames):
     D, rx, vox, arr = random_case(rng, T=2, E=9, S=48, F=frames, V=150)
     d, r, v, a = to_dev(D, rx, vox, arr, device=device)
     outs = {}
-    for lay in autotune.candidates(frames):
+    for lay in autotune.candidates(frames, band=False):
         out = torch.zeros((150, frames), dtype=torch.complex64, device=device)
         _hip_impl.beamform_compound(d, r, v, a, out, 0.8, -1e-7, 20e6, 1540.0, 5e6, 0.5, 2, layout=lay)
         outs[lay] = out
@@ -283,3 +283,49 @@ def test_every_layout_gives_identical_results(device, frames):
     assert rel_l2(first.cpu().numpy(), ref) <= TOL_FP32
     for lay, o in outs.items():
         assert torch.equal(o, first), hex(lay)
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2])
+@pytest.mark.parametrize("kind", ["iq", "iq16", "rf"])
+@pytest.mark.parametrize("frames", [64, 128, 256])
+def test_band_kernel_matches_plain_and_oracle(device, interp, kind, frames):
+    """LDS-staged band kernel (selected by a band layout code) vs the general kernel with a forced
+    layout: bitwise equal; and vs the oracle on a voxel sample. A regular depth-major
+    grid keeps windows in LDS; the random cloud forces the global fallback for wide windows."""
+    from mach_amd import _hip_impl, autotune
+
+    log(f"band interp={interp} kind={kind} frames={frames}")
+    acq = phantoms.doppler(device=device, scale=0.25, n_frames=frames)
+    rng = np.random.default_rng(frames + interp)
+    cloud = torch.as_tensor((rng.normal(size=(4200, 3)) * [8e-3, 0, 10e-3] + [0, 0, 25e-3]).astype(np.float32),
+                            device=device)
+    cloud_arr = (cloud[:, 2] / 1540.0)[None].contiguous()
+    for scan, arr in ((acq.scan_coords_m, acq.tx_wave_arrivals_s), (cloud, cloud_arr)):
+        cd = acq.channel_data
+        if kind == "iq16":
+            cd = acq.fp16_channel_data()
+        elif kind == "rf":
+            cd = acq.channel_data.real.contiguous()
+        otype = torch.float32 if kind == "rf" else torch.complex64
+        fmod = 0.0 if kind == "rf" else acq.params["modulation_freq_hz"]
+        args = (acq.params["f_number"], 0.0, acq.params["sampling_freq_hz"], acq.params["sound_speed_m_s"], fmod, 0.5,
+                interp)
+        band = torch.zeros((scan.shape[0], frames), dtype=otype, device=device)
+        plain = torch.zeros_like(band)
+        for k in (4, 2, 1):  # widest band layout this dtype / frame count accepts
+            try:
+                _hip_impl.beamform_compound(cd, acq.rx_coords_m, scan, arr, band, *args, layout=autotune.encode(0, k))
+                break
+            except RuntimeError:
+                continue
+        else:
+            raise AssertionError("no band layout accepted")
+        _hip_impl.beamform_compound(cd, acq.rx_coords_m, scan, arr, plain, *args, layout=autotune.encode(64, 1))
+        torch.cuda.synchronize()
+        assert torch.equal(band, plain)
+        idx = np.sort(rng.choice(scan.shape[0], 300, replace=False))
+        ref = das_c(cd.cpu().numpy(), acq.rx_coords_m.cpu().numpy(), scan.cpu().numpy()[idx], arr.cpu().numpy()[:, idx],
+                    f_number=args[0], rx_start_s=0.0, sampling_freq_hz=args[2], sound_speed_m_s=args[3],
+                    modulation_freq_hz=fmod, tukey_alpha=0.5, interp_type=interp)
+        assert np.abs(ref).max() > 0
+        assert rel_l2(band.cpu().numpy()[idx], ref) <= TOL_FP32
