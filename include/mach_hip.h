@@ -72,9 +72,15 @@ typedef struct mach_beamform_args {
     int32_t accumulate;               /* 1: out += result (reference device-array semantics,
                                          kernel.cu:674-680); 0: out = result                    */
     int32_t layout;                   /* 0: automatic. Otherwise (lanes_per_voxel << 4) |
-                                         frames_per_lane, e.g. 0x404: 64 lanes share a voxel,
+                                         frames_per_lane, e.g. 0x104: 16 lanes share a voxel,
                                          4 frames each (the autotuner's choice; rejected if it
-                                         does not divide n_frames or break vector alignment)    */
+                                         does not divide n_frames or break vector alignment).
+                                         lanes_per_voxel 0 (0x001, 0x002, 0x004) selects the
+                                         LDS-staged kernel for frame ensembles, which needs a
+                                         voxel_order and n_frames a multiple of 64 x
+                                         frames_per_lane. Every layout gives bit-identical
+                                         results for single-transmit requests; general-kernel
+                                         layouts also for several transmits.                   */
     const int32_t *voxel_order;       /* optional device [n_vox] permutation of 0..n_vox-1: the
                                          order voxels are processed in (NULL = 0, 1, 2, ...).
                                          Results still land at out[v]; a depth-major order
@@ -105,8 +111,8 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
 int mach_beamform(const mach_beamform_args *args, void *stream);
 
 /* Batched execution: many requests (any mix of shapes, data types, interpolation types and
- * transmit counts) run in ONE persistent-kernel launch that drains a device work queue, replayed
- * from a captured hipGraph. No reference counterpart: the reference runs one synchronous launch per
+ * transmit counts) run in one submission of two persistent kernels (single-transmit requests, then
+ * multi-transmit ones) that drain a device work queue, replayed from a captured hipGraph. No reference counterpart: the reference runs one synchronous launch per
  * request and transmit (src/mach/kernel.cu:820-874, src/mach/experimental.py:80-97).
  * The batch object owns two descriptor slots, so two submissions can be in flight; a third waits
  * for the oldest. Requests' arrays must stay valid until the submission's work has completed on
@@ -114,10 +120,12 @@ int mach_beamform(const mach_beamform_args *args, void *stream);
 typedef struct mach_batch mach_batch;
 
 /* Allocates the batch object (pinned + device descriptor tables for up to `max_requests` each).
- * use_graph != 0 replays a captured hipGraph (copy descriptors -> reset queue -> persistent kernel). */
+ * use_graph != 0 replays a captured hipGraph (copy descriptors -> reset queues -> persistent
+ * kernels). */
 int mach_batch_create(uint32_t max_requests, int32_t use_graph, mach_batch **batch);
 
-/* Enqueues all `n_requests` requests on `stream` as one persistent launch. */
+/* Enqueues all `n_requests` requests on `stream` as one submission. The band layouts run as the
+ * general kernel here. */
 int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uint32_t n_requests,
                       void *stream);
 

@@ -75,10 +75,10 @@ def time_layout(run, layout: int, reps: int = 5, warmup: int = 1) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(run, n_frames: int, reps: int = 5):
+def tune(run, n_frames: int, reps: int = 5, band: bool = True):
     """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms})."""
     times = {}
-    for lay in candidates(n_frames):
+    for lay in candidates(n_frames, band=band):
         try:
             times[lay] = time_layout(run, lay, reps)
         except RuntimeError:  # layout rejected for this request (alignment)
@@ -160,7 +160,9 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = Tr
                                     p["modulation_freq_hz"], p["tukey_alpha"], p["interp_type"], accumulate=False,
                                     layout=layout)
 
-    best, times = tune(run, acq.n_frames, reps)
+    # the band kernel sums transmits in a different order: keep it to single-transmit requests so that
+    # a tuned layout never changes results (all general-kernel layouts are bit-identical)
+    best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1)
     key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox)
     if record:
         table().record(key, best, times, {"workload": acq.name, "storage": storage})

@@ -240,14 +240,18 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
 // order) and frame chunk `by` (LF * FPL frames). Used by the plain kernel and the batched kernel.
 // Transmits are taken TC at a time: the transmit-independent pair geometry (aperture, receive delay,
-// apodisation) is computed once per chunk and reused for its TC transmits. With TC = 1 the per-voxel
-// summation order is (transmit, element); with TC > 1 it is (transmit chunk, element round, transmit,
-// element in round).
+// apodisation) is computed once per chunk and reused for its TC transmits. The per-voxel summation
+// order is (transmit chunk, element, transmit within chunk) whatever the layout.
 template <int ST, int INTERP, int LF, int FPL, int TC = 1>
 __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
     constexpr int LV = 64 / LF;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // The empty asm makes the thread index opaque, so nothing derived from it is hoisted out of a
+    // caller's loop. In the persistent batched kernel (which runs every layout variant inside one
+    // loop) hoisting kept the lane constants of all variants live at once: 144 instead of 122 VGPRs.
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lv = lane / LF;
     const int lf = lane % LF;
     const int64_t vgroup = (int64_t)bx * 4 + wave;
@@ -286,18 +290,36 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
                 if (tt < nt) s[tt] = pair_time<INTERP>(p, g, Tv[tt]);
             }
-            const int nj = (int)((p.n_rx - e0) < LF ? (p.n_rx - e0) : LF);
+            // Only elements that some voxel of this wave uses under some transmit of the chunk: fold
+            // the wave's ballot (bit lv * LF + j: lane lv * LF + j computed element e0 + j of voxel lv)
+            // onto the LF element slots and walk the set bits with a scalar loop. The predicate is the
+            // pair's validity only: the lane that computed a pair may itself own no frames.
+            uint64_t jm = 0ull;
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-                if (tt >= nt) break;
-                const int64_t tx_base = (t0 + tt) * p.n_rx;
-                for (int j = 0; j < nj; ++j) {
+            for (int tt = 0; tt < TC; ++tt) jm |= __ballot(s[tt].i0 >= 0);
+            if constexpr (LF < 64) {
+#pragma unroll
+                for (int sh = LF; sh < 64; sh <<= 1) jm |= jm >> sh;
+                jm &= (1ull << LF) - 1ull;
+            }
+            // Summation order per voxel: (transmit chunk, element, transmit) -- the same for every
+            // layout, so all layouts (and the batched kernel) give bit-identical results.
+            while (jm != 0ull) {
+                const int j = __builtin_ctzll(jm);
+                jm &= jm - 1ull;
+                const int64_t e_abs = e0 + j;
+#pragma unroll
+                for (int tt = 0; tt < TC; ++tt) {
+                    if (tt >= nt) break;
                     const int i0 = bcast<LF>(s[tt].i0, j, lane);
                     const float frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
                     const float wc = __int_as_float(bcast<LF>(__float_as_int(s[tt].wc), j, lane));
                     const float ws = __int_as_float(bcast<LF>(__float_as_int(s[tt].ws), j, lane));
                     if (i0 >= 0 && fok) {
-                        const int64_t off = ((tx_base + e0 + j) * p.n_samples + i0) * p.n_frames + f0;
+                        const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;  // uniform
+                        const int64_t off = row0 + f0 +
+                                            (p.rows32 ? (int64_t)((uint32_t)i0 * (uint32_t)p.n_frames)
+                                                      : (int64_t)i0 * p.n_frames);
                         accumulate_pair<ST, INTERP, FPL>(p.data, off, p.tap_step, frac, wc, ws, acc_re, acc_im);
                     }
                 }

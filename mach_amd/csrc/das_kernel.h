@@ -34,7 +34,7 @@ struct DasParams {
     int32_t rotate;          // IQ data with fmod != 0
     int32_t accumulate;      // out += result (1) or out = result (0)
     int32_t apod;            // alpha > 0
-    int32_t pad0;
+    int32_t rows32;          // n_samples * n_frames < 2^32: sample offsets within a channel fit 32 bits
 };
 
 // Frame-lane layout of a launch: LF lanes of a wave share one voxel and split its frames, each lane

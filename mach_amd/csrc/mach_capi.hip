@@ -86,7 +86,7 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->rotate = (a->data_type != MACH_DATA_F32 && a->modulation_freq_hz != 0.0f) ? 1 : 0;
     p->accumulate = a->accumulate ? 1 : 0;
     p->apod = a->tukey_alpha > 0.0f ? 1 : 0;  // kernel.cu:749
-    p->pad0 = 0;
+    p->rows32 = (a->n_samples * a->n_frames) < (1ull << 32) ? 1 : 0;
     return MACH_OK;
 }
 

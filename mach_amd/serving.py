@@ -10,8 +10,10 @@ server:
 * admits arrived requests in arrival order while their channel-data and output bytes fit the HBM
   budget (the rings' capacity). Later arrivals wait until completed work frees space;
 * packs every admitted, not-yet-running request into one persistent-kernel launch (``BatchRunner``,
-  graph replay). Up to two batches are in flight, on two streams, so requests that arrive while a
-  batch runs are admitted and start while the previous batch drains;
+  graph replay). Requests of a batch that share a geometry (probe, grid, transmits, dtype) are fused
+  into one ensemble along the frame axis, so their float64 pair geometry is computed once. Up to two
+  batches are in flight, on two streams, so requests that arrive while a batch runs are admitted and
+  start while the previous batch drains;
 * reports per-request latency (arrival -> result ready on the device), p50/p99, and throughput in
   points/s over the makespan.
 
@@ -30,6 +32,7 @@ import time
 import numpy as np
 import torch
 
+from . import autotune
 from . import batch as _batch
 from . import phantoms
 from ._hip_impl import voxel_order
@@ -214,11 +217,14 @@ def _data_view(req: ServeRequest, ring: DeviceRing, slot):
 def _to_request(req: ServeRequest, data: torch.Tensor, out: torch.Tensor) -> _batch.Request:
     g = req.geometry
     p = g.params
-    return _batch.Request(data, g.rx_coords_m, g.scan_coords_m, g.tx_wave_arrivals_s, out,
-                          f_number=p["f_number"], rx_start_s=p["rx_start_s"], sampling_freq_hz=p["sampling_freq_hz"],
-                          sound_speed_m_s=p["sound_speed_m_s"], modulation_freq_hz=p["modulation_freq_hz"],
-                          tukey_alpha=p["tukey_alpha"], interp_type=p["interp_type"], accumulate=False,
-                          voxel_order=g.voxel_order)
+    r = _batch.Request(data, g.rx_coords_m, g.scan_coords_m, g.tx_wave_arrivals_s, out,
+                       f_number=p["f_number"], rx_start_s=p["rx_start_s"], sampling_freq_hz=p["sampling_freq_hz"],
+                       sound_speed_m_s=p["sound_speed_m_s"], modulation_freq_hz=p["modulation_freq_hz"],
+                       tukey_alpha=p["tukey_alpha"], interp_type=p["interp_type"], accumulate=False,
+                       voxel_order=g.voxel_order)
+    r.layout = autotune.layout_for(r.kind(), r.interp_type, r.n_tx(), g.rx_coords_m.shape[0], out.shape[1],
+                                   g.scan_coords_m.shape[0], data.data_ptr(), out.data_ptr())
+    return r
 
 
 @dataclasses.dataclass
@@ -232,6 +238,7 @@ class ServeReport:
     mean_ms: float
     max_batch: int = 1
     n_batches: int = 0
+    n_fused: int = 0  # requests that ran fused with others of the same geometry
 
     @property
     def gpts_per_s(self) -> float:
@@ -251,9 +258,30 @@ def _report(mode, reqs, t_first, max_batch=1, n_batches=0) -> ServeReport:
                        max_batch, n_batches)
 
 
-def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False):
-    """Continuous batching over the persistent kernel. Returns a ServeReport; with ``keep_outputs`` each
-    request's ``out`` is a private copy of its result."""
+def _fusion_groups(reqs, fuse=True, max_frames=256):
+    """Groups of requests that can run as one ensemble: same geometry object (probe, grid, transmits,
+    parameters), storage dtype and [n_tx, n_rx, n_samples]; frames concatenated up to ``max_frames``."""
+    if not fuse:
+        return [[r] for r in reqs]
+    groups, open_ = [], {}
+    for r in reqs:
+        key = (id(r.geometry), r.source.dtype, tuple(r.source.shape[:3]), tuple(r.source.shape[4:]))
+        g = open_.get(key)
+        nf = r.source.shape[3]
+        if g is not None and sum(x.source.shape[3] for x in g) + nf <= max_frames:
+            g.append(r)
+        else:
+            g = [r]
+            open_[key] = g
+            groups.append(g)
+    return groups
+
+
+def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
+                  fuse=True, max_fused_frames=256):
+    """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
+    fused into one ensemble (``fuse``). Returns a ServeReport; with ``keep_outputs`` each request's
+    ``out`` is a private copy of its result."""
     free, _total = torch.cuda.mem_get_info(device)
     budget = min(int(budget_bytes), int(free * 0.8))
     data_ring = DeviceRing(budget * 3 // 4, device)
@@ -265,7 +293,7 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     waiting = collections.deque()
     admitted = []  # (req, copy_event)
     inflight = collections.deque()  # (reqs, event)
-    n_done, n_batches, biggest = 0, 0, 0
+    n_done, n_batches, biggest, n_fused = 0, 0, 0, 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     while n_done < len(reqs):
@@ -291,7 +319,7 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             admitted.append((r, ev))
         # completions (batches finish in order)
         while inflight and inflight[0][1].query():
-            done_reqs, _ = inflight.popleft()
+            done_reqs, _, _ = inflight.popleft()
             t_done = time.perf_counter() - t0
             for r in done_reqs:
                 r.done_s = t_done
@@ -308,14 +336,30 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             stream = compute[n_batches % 2]
             for _r, ev in take:
                 stream.wait_event(ev)
-            breqs = []
-            for r, _ev in take:
-                r.out = _out_view(r, out_ring, r.out_slot)
-                breqs.append(_to_request(r, _data_view(r, data_ring, r.data_slot), r.out))
+            breqs, keep = [], []
+            for group in _fusion_groups([r for r, _ in take], fuse, max_fused_frames):
+                if len(group) == 1:
+                    r = group[0]
+                    r.out = _out_view(r, out_ring, r.out_slot)
+                    breqs.append(_to_request(r, _data_view(r, data_ring, r.data_slot), r.out))
+                    continue
+                # same geometry: one ensemble request over the concatenated frames
+                with torch.cuda.stream(stream):
+                    fused = torch.cat([_data_view(r, data_ring, r.data_slot) for r in group], dim=3)
+                    out_f = torch.empty((group[0].geometry.scan_coords_m.shape[0], fused.shape[3]),
+                                        dtype=torch.complex64, device=device)
+                a = 0
+                for r in group:
+                    nf = r.source.shape[3]
+                    r.out = out_f[:, a:a + nf]
+                    a += nf
+                breqs.append(_to_request(group[0], fused, out_f))
+                keep.append(fused)
+                n_fused += len(group)
             runner.submit(breqs, stream=stream)
             ev = torch.cuda.Event()
             ev.record(stream)
-            inflight.append(([r for r, _ in take], ev))
+            inflight.append(([r for r, _ in take], ev, keep))
             n_batches += 1
             biggest = max(biggest, len(take))
         elif not admitted and not inflight and pending and not waiting:
@@ -326,6 +370,7 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     torch.cuda.synchronize()
     runner.close()
     rep = _report("batched", reqs, reqs[0].arrival_s if reqs else 0.0, biggest, n_batches)
+    rep.n_fused = n_fused
     del data_ring, out_ring
     return rep
 

@@ -17,25 +17,47 @@ def stream_reqs(device):
                                frames=(1, 2, 4, 8, 16, 32, 64))
 
 
+@pytest.fixture(scope="module")
+def burst_reqs(device):
+    # every request arrives within a few microseconds: the server must batch (and fuse)
+    return serving.make_stream(device, n_requests=24, qps=1e7, seed=4, template_scale=0.25,
+                               frames=(1, 2, 4, 8, 16, 32, 64))
+
+
 def _reset(reqs):
     for r in reqs:
         r.done_s = r.admit_s = float("nan")
         r.out = None
 
 
-def test_mix_equals_alone(device, stream_reqs):
-    _reset(stream_reqs)
-    plain = serving.serve_plain(stream_reqs, device, keep_outputs=True)
-    alone = {r.rid: r.out for r in stream_reqs}
-    _reset(stream_reqs)
-    mixed = serving.serve_batched(stream_reqs, device, budget_bytes=8 << 30, keep_outputs=True)
-    for r in stream_reqs:
+@pytest.mark.parametrize("fuse", [True, False])
+def test_mix_equals_alone(device, burst_reqs, fuse):
+    reqs = burst_reqs
+    _reset(reqs)
+    plain = serving.serve_plain(reqs, device, keep_outputs=True)
+    alone = {r.rid: r.out for r in reqs}
+    _reset(reqs)
+    mixed = serving.serve_batched(reqs, device, budget_bytes=8 << 30, keep_outputs=True, fuse=fuse)
+    for r in reqs:
         assert torch.equal(r.out, alone[r.rid]), r.rid
     print(plain.as_dict(), mixed.as_dict(), flush=True)
-    assert mixed.n_requests == plain.n_requests == len(stream_reqs)
-    assert mixed.n_batches < len(stream_reqs)  # requests were actually batched
+    assert mixed.n_requests == plain.n_requests == len(reqs)
+    assert mixed.n_batches < len(reqs)  # requests were actually batched
+    assert (mixed.n_fused > 0) == fuse
     for rep in (plain, mixed):
         assert math.isfinite(rep.p50_ms) and rep.p99_ms >= rep.p50_ms > 0
+
+
+def test_poisson_stream_equals_alone(device, stream_reqs):
+    # arrivals spread over time: small batches, two in flight
+    _reset(stream_reqs)
+    serving.serve_plain(stream_reqs, device, keep_outputs=True)
+    alone = {r.rid: r.out for r in stream_reqs}
+    _reset(stream_reqs)
+    rep = serving.serve_batched(stream_reqs, device, budget_bytes=8 << 30, keep_outputs=True)
+    assert rep.n_requests == len(stream_reqs)
+    for r in stream_reqs:
+        assert torch.equal(r.out, alone[r.rid]), r.rid
 
 
 def test_budget_backpressure_keeps_results(device, stream_reqs):

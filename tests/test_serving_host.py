@@ -41,3 +41,20 @@ def test_ring_undo_and_limits():
         ring.alloc(4096)
     view = ring.view((0, 512), __import__("torch").float32, (4, 32))
     assert view.shape == (4, 32)
+
+
+def test_fusion_groups_by_geometry_and_frame_cap():
+    import torch
+
+    from mach_amd.serving import ServeRequest, _fusion_groups
+
+    ga, gb = object(), object()
+
+    def req(rid, geo, frames, dtype=torch.complex64):
+        return ServeRequest(rid, 0.0, geo, torch.empty((1, 4, 8, frames), dtype=dtype), "x", 0)
+
+    rs = [req(0, ga, 100), req(1, gb, 8), req(2, ga, 100), req(3, ga, 100), req(4, gb, 8),
+          req(5, ga, 4, torch.float32)]
+    groups = [[r.rid for r in g] for g in _fusion_groups(rs, True, 256)]
+    assert groups == [[0, 2], [1, 4], [3], [5]]  # frame cap 256, dtype separates
+    assert [[r.rid for r in g] for g in _fusion_groups(rs, False)] == [[i] for i in range(6)]
