@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,28 +36,51 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(acq, storage: str, accumulate: bool = False) -> int:
+def algorithmic_bytes(acq, storage: str, accumulate: bool = False, output: str = "signal") -> int:
     T, E, S, F, V = acq.n_tx, acq.n_rx, acq.n_samples, acq.n_frames, acq.n_vox
     data = T * E * S * F * (4 if storage == "fp16" else 8)
     coords = (E + V) * 3 * 4
     arrivals = T * V * 4
-    out = V * F * 8 * (2 if accumulate else 1)
+    out = {"signal": V * F * 8, "power": V * 4, "envelope": V * F * 4}[output] * (2 if accumulate else 1)
     return data + coords + arrivals + out
 
 
-def run_gpu(acq, storage, steps, warmup):
+def measured_traffic(workload: str, storage: str, output: str):
+    """HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE counters (calibrated on a copy kernel)
+    for this workload, from its committed counters summary; None when there is none."""
+    suffix = "" if output == "signal" else f"_{output}"
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"das_kernel_{workload}_{storage}{suffix}_counters.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", fh.read())
+    return (int(m.group(1)), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))) if m else (None, None)
+
+
+def run_gpu(acq, storage, steps, warmup, output="signal"):
     import mach_amd as mach
-    from mach_amd import _hip_impl
+    from mach_amd import _hip_impl, postprocess
 
     p = dict(acq.params)
     cd = acq.fp16_channel_data() if storage == "fp16" else acq.channel_data
-    out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
     interp = mach.InterpolationType(p["interp_type"])
+    if output == "signal":
+        out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
 
-    def step():
-        _hip_impl.beamform_compound(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
-                                    p["f_number"], p["rx_start_s"], p["sampling_freq_hz"], p["sound_speed_m_s"],
-                                    p["modulation_freq_hz"], p["tukey_alpha"], interp, accumulate=False)
+        def step():
+            _hip_impl.beamform_compound(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
+                                        p["f_number"], p["rx_start_s"], p["sampling_freq_hz"], p["sound_speed_m_s"],
+                                        p["modulation_freq_hz"], p["tukey_alpha"], interp, accumulate=False)
+    else:
+        fn = postprocess.power_doppler if output == "power" else postprocess.envelope
+        shape = (acq.n_vox,) if output == "power" else (acq.n_vox, acq.n_frames)
+        out = torch.empty(shape, dtype=torch.float32, device=cd.device)
+        kw = {k: p[k] for k in ("f_number", "rx_start_s", "sampling_freq_hz", "sound_speed_m_s",
+                                "modulation_freq_hz", "tukey_alpha")}
+
+        def step():
+            fn(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out, interp_type=interp, **kw)
 
     for _ in range(warmup):
         step()
@@ -151,6 +175,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="doppler", choices=["doppler", "plane_wave", "compound", "volumetric"])
     ap.add_argument("--storage", default=None, choices=["c64", "fp16"])
+    ap.add_argument("--output", default="signal", choices=["signal", "power", "envelope"],
+                    help="what the kernel writes: the per-frame signal (the reference's output, default), the "
+                         "power Doppler image or the envelope (fused epilogues)")
     ap.add_argument("--cpu-voxels", type=int, default=0, help="voxel sample for the CPU baseline (0 = all)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -178,10 +205,11 @@ def main(argv=None):
     acq = phantoms.make(args.workload, device=dev)
     torch.cuda.synchronize()
     log(f"bench: shape T,E,S,F={tuple(acq.channel_data.shape)} voxels={acq.n_vox} storage={storage}")
-    ms_step, kernel_ms, _ = run_gpu(acq, storage, args.steps, args.warmup)
+    ms_step, kernel_ms, _ = run_gpu(acq, storage, args.steps, args.warmup, args.output)
     pts = acq.points
     value = pts / (ms_step * 1e-3) / 1e9
-    b_alg = algorithmic_bytes(acq, storage)
+    b_alg = algorithmic_bytes(acq, storage, output=args.output)
+    traffic, traffic_src = measured_traffic(args.workload, storage, args.output)
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     interp = int(acq.params["interp_type"])
     valu_tflops = FLOP_PER_POINT[interp] * pts / (kernel_ms * 1e-3) / 1e12
@@ -208,6 +236,7 @@ def main(argv=None):
                 "volumetric": "32x32 matrix, 5 diverging tx x 1024 ch x 1536 samples x 8 frames, 128^3 voxels, linear, f#1.5",
             }[args.workload],
             "storage": storage,
+            "output": args.output,
             "points_per_step": pts,
             "parallelism": "single GPU",
         },
@@ -217,7 +246,8 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": b_alg,
             "kernel_ms": round(kernel_ms, 4),
         },

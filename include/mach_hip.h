@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define MACH_ABI_VERSION 1
+#define MACH_ABI_VERSION 2
 
 /* Interpolation selector; same integer values as the reference's
  * `enum class InterpolationType` (src/mach/kernel.cu:25-29, exported at kernel.cu:1222-1226). */
@@ -36,6 +36,17 @@ typedef enum mach_data_type {
     MACH_DATA_C64 = 1,   /* IQ complex64 in, complex64 out                   */
     MACH_DATA_C32H = 2   /* IQ as (binary16 re, binary16 im) in, complex64 out */
 } mach_data_type;
+
+/* What a beamform call writes to `out`. The reference returns the beamformed signal and leaves
+ * post-processing to the caller: power Doppler as np.square(np.abs(y)).sum(axis=-1)
+ * (examples/doppler.py:252, tests/compare/test_pymust.py:317-318) and B-mode as 20 log10 |y|
+ * normalised to its maximum (src/mach/_vis.py:17-40). The epilogues compute these inside the kernel,
+ * so the per-frame signal never reaches HBM. */
+typedef enum mach_output_kind {
+    MACH_OUTPUT_SIGNAL = 0,   /* out [n_vox][n_frames]: complex64 for IQ, float32 for RF (reference) */
+    MACH_OUTPUT_POWER = 1,    /* out float32 [n_vox]: sum over frames of |y|^2 (power Doppler)       */
+    MACH_OUTPUT_ENVELOPE = 2  /* out float32 [n_vox][n_frames]: |y| (B-mode envelope)              */
+} mach_output_kind;
 
 typedef enum mach_status {
     MACH_OK = 0,
@@ -86,6 +97,14 @@ typedef struct mach_beamform_args {
                                          Results still land at out[v]; a depth-major order
                                          (sorted by z, then x, then y) keeps the sample windows
                                          of neighbouring waves in the same L2.                   */
+    int32_t output;                   /* mach_output_kind (0 = the reference's signal output).
+                                         POWER accepts accumulate (adds to out); ENVELOPE needs
+                                         accumulate = 0. Epilogues run in the general kernel
+                                         only (not the LDS-staged one, not in batches).          */
+    float *peak;                      /* optional device float for POWER / ENVELOPE: receives the
+                                         maximum value written to out (set to 0 first, then
+                                         raised with an order-independent atomic max). Calls
+                                         that share one `peak` must not overlap.                 */
 } mach_beamform_args;
 
 /* Complex64 (IQ) entry. Replaces the nanobind overload `_cuda_impl.beamform` for complex64
@@ -109,6 +128,13 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
 /* General entry (transmit compounding, FP16 IQ storage, overwrite or accumulate). Replaces
  * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209. */
 int mach_beamform(const mach_beamform_args *args, void *stream);
+
+/* In-place logarithmic compression of a POWER or ENVELOPE output: x[i] = k log10(a(x[i])) -
+ * k log10(a(*peak)) with a(x) = |x|, or float64 machine epsilon where x == 0, and k = 10 for power
+ * (power_mode != 0) or 20 for amplitude. Restates `db_zero` of src/mach/_vis.py:17-40 (which
+ * normalises by the maximum of the converted image) for a device image whose maximum is `*peak`.
+ * `peak` is a device pointer; n may be 0. No reference counterpart as a library function. */
+int mach_to_db(float *x, uint64_t n, const float *peak, int32_t power_mode, void *stream);
 
 /* Batched execution: many requests (any mix of shapes, data types, interpolation types and
  * transmit counts) run in one submission of two persistent kernels (single-transmit requests, then

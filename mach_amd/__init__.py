@@ -3,13 +3,15 @@
 Drop-in for the reference package ``mach`` (``src/mach/__init__.py:10-31``): ``beamform``,
 ``kernel``, ``experimental``, ``wavefront`` and ``geometry`` keep the reference's names and
 behaviour; the compute runs in hand-written HIP kernels behind the C ABI of ``include/mach_hip.h``.
-``import mach_amd as mach`` is the intended use.
+``import mach_amd as mach`` is the intended use. Beyond the reference: ``postprocess`` (power
+Doppler and B-mode images computed in the kernel's epilogue), ``batch`` (many requests per persistent
+launch), ``serving`` (continuous batching), ``autotune`` and ``phantoms`` (synthetic workloads).
 """
 
 __version__ = "0.1.3"
 
 from . import geometry, probes, wavefront  # noqa: E402  (host-only helpers, no native code)
-from . import experimental, kernel  # noqa: E402  (load the native library; fail loudly if missing)
+from . import experimental, kernel, postprocess  # noqa: E402  (load the native library; fail loudly if missing)
 from .kernel import InterpolationType, beamform  # noqa: E402
 
 __all__ = [
@@ -19,6 +21,7 @@ __all__ = [
     "experimental",
     "geometry",
     "kernel",
+    "postprocess",
     "probes",
     "wavefront",
 ]

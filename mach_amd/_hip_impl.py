@@ -177,18 +177,24 @@ def _device_for(tensors):
     return torch.device("cuda", torch.cuda.current_device())
 
 
+# mach_output_kind
+OUTPUT_SIGNAL, OUTPUT_POWER, OUTPUT_ENVELOPE = 0, 1, 2
+
+
 def make_args(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
               sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate=True, layout=0,
-              order=None) -> "_lib.BeamformArgs":
-    """``mach_beamform_args`` for device tensors (shapes already checked)."""
+              order=None, output=OUTPUT_SIGNAL, peak=None) -> "_lib.BeamformArgs":
+    """``mach_beamform_args`` for device tensors (shapes already checked). The frame count is the
+    channel data's."""
     core = _core_shape(cd, kind)
     return _lib.BeamformArgs(
         cd.data_ptr(), rx.data_ptr(), scan.data_ptr(), arrivals.data_ptr(), out.data_ptr(),
-        n_tx, rx.shape[0], core[-2], out.shape[1], scan.shape[0],
+        n_tx, rx.shape[0], core[-2], core[-1], scan.shape[0],
         kind, int(interp_type),
         f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s, modulation_freq_hz, tukey_alpha,
         1 if accumulate else 0, int(layout),
         order.data_ptr() if order is not None else None,
+        int(output), peak.data_ptr() if peak is not None else None,
     )
 
 

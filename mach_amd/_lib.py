@@ -23,6 +23,8 @@ STATUS_NAMES = {
     5: "MACH_ERR_UNSUPPORTED",
 }
 
+ERR_INVALID_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED = 1, 2, 3, 4, 5
+
 # Data-type codes (mach_data_type)
 DATA_F32, DATA_C64, DATA_C32H = 0, 1, 2
 
@@ -52,6 +54,8 @@ class BeamformArgs(ctypes.Structure):
         ("accumulate", ctypes.c_int32),
         ("layout", ctypes.c_int32),
         ("voxel_order", ctypes.c_void_p),
+        ("output", ctypes.c_int32),
+        ("peak", ctypes.c_void_p),
     ]
 
 
@@ -71,6 +75,8 @@ SIGNATURES = {
     "mach_batch_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BeamformArgs), ctypes.c_uint32,
                                          ctypes.c_void_p]),
     "mach_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "mach_to_db": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int32,
+                                  ctypes.c_void_p]),
     "mach_last_error": (ctypes.c_char_p, []),
     "mach_abi_version": (ctypes.c_int32, []),
     "mach_hip_runtime_version": (ctypes.c_char_p, []),

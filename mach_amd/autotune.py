@@ -26,12 +26,22 @@ FRAMES_PER_LANE = (1, 2, 4)
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "gfx950.json")
 
 
-def encode(lanes_per_voxel: int, frames_per_lane: int) -> int:
-    return (lanes_per_voxel << 4) | frames_per_lane
+# (lanes_per_voxel, frames_per_lane) layouts also compiled with 2 or 4 frame groups per lane
+MULTI_GROUP = {(16, 4), (32, 2), (64, 1), (8, 4)}
+
+
+def encode(lanes_per_voxel: int, frames_per_lane: int, chunks_per_lane: int = 1) -> int:
+    """C-ABI layout code: (log2(chunks_per_lane) << 12) | (lanes_per_voxel << 4) | frames_per_lane."""
+    return ((chunks_per_lane.bit_length() - 1) << 12) | (lanes_per_voxel << 4) | frames_per_lane
 
 
 def decode(layout: int):
-    return (layout >> 4) & 0xFFF, layout & 0xF
+    """(lanes_per_voxel, frames_per_lane) of a layout code."""
+    return (layout >> 4) & 0xFF, layout & 0xF
+
+
+def chunks_per_lane(layout: int) -> int:
+    return 1 << ((layout >> 12) & 0xF)
 
 
 def _bucket(n: int) -> int:
@@ -39,8 +49,10 @@ def _bucket(n: int) -> int:
     return 1 << max(0, math.ceil(math.log2(max(1, n))))
 
 
-def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int) -> str:
-    return f"d{kind}:i{int(interp)}:t{_bucket(n_tx)}:e{_bucket(n_rx)}:f{n_frames}:v{_bucket(n_vox)}"
+def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int, output: int = 0) -> str:
+    """Shape class of a request; ``output`` (mach_output_kind) other than the signal gets its own entry."""
+    key = f"d{kind}:i{int(interp)}:t{_bucket(n_tx)}:e{_bucket(n_rx)}:f{n_frames}:v{_bucket(n_vox)}"
+    return key + (f":o{int(output)}" if output else "")
 
 
 def candidates(n_frames: int, band: bool = True):
@@ -58,6 +70,9 @@ def candidates(n_frames: int, band: bool = True):
             # lanes beyond the frames only help setup-bound problems (few frames): allow up to 8
             if lf * k <= max(2 * n_frames, 8) and (lf * k >= min(n_frames, 64) // 4 or lf == 1):
                 out.append(encode(lf, k))
+            # several frame groups per lane share one setup when the frames fill them
+            if (lf, k) in MULTI_GROUP:
+                out += [encode(lf, k, ch) for ch in (2, 4) if lf * k * ch <= n_frames]
     return out
 
 
@@ -132,40 +147,54 @@ def table() -> TuningTable:
 
 
 def layout_for(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int,
-               data_ptr: int = 0, out_ptr: int = 0) -> int:
+               data_ptr: int = 0, out_ptr: int = 0, output: int = 0) -> int:
     """Tuned layout for a request, or 0 (let the library choose) when the shape class has no entry or
     the tuned frames-per-lane does not fit this call's pointer alignment."""
-    lay = table().lookup(shape_key(kind, interp, n_tx, n_rx, n_frames, n_vox))
+    lay = table().lookup(shape_key(kind, interp, n_tx, n_rx, n_frames, n_vox, output))
     if lay:
         _, k = decode(lay)
         elem = 8 if kind == 1 else 4
-        out_elem = 4 if kind == 0 else 8
+        out_elem = (4 if kind == 0 else 8) if output == 0 else 1  # epilogues store float32 scalars
         if n_frames % k or data_ptr % (k * elem) or out_ptr % (k * out_elem):
             return 0
     return lay
 
 
-def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = True):
-    """Tunes the layout for a synthetic acquisition (``mach_amd.phantoms.Acquisition``)."""
-    from . import _hip_impl
+def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = True, output: int = 0):
+    """Tunes the layout for a synthetic acquisition (``mach_amd.phantoms.Acquisition``) and one output
+    kind (mach_output_kind: 0 signal, 1 power Doppler image, 2 envelope)."""
+    import ctypes
+
+    from . import _hip_impl, _lib
 
     p = acq.params
     cd = acq.fp16_channel_data() if storage == "fp16" else acq.channel_data
     kind = _hip_impl._data_kind(cd)
-    out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
+    if output == 1:
+        out = torch.empty((acq.n_vox,), dtype=torch.float32, device=cd.device)
+    elif output == 2:
+        out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.float32, device=cd.device)
+    else:
+        out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
+    order = _hip_impl.voxel_order(acq.scan_coords_m)
+    lib = _lib.load()
 
     def run(layout):
-        _hip_impl.beamform_compound(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
-                                    p["f_number"], p["rx_start_s"], p["sampling_freq_hz"], p["sound_speed_m_s"],
-                                    p["modulation_freq_hz"], p["tukey_alpha"], p["interp_type"], accumulate=False,
-                                    layout=layout)
+        args = _hip_impl.make_args(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
+                                   n_tx=acq.n_tx, kind=kind, f_number=p["f_number"], rx_start_s=p["rx_start_s"],
+                                   sampling_freq_hz=p["sampling_freq_hz"], sound_speed_m_s=p["sound_speed_m_s"],
+                                   modulation_freq_hz=p["modulation_freq_hz"], tukey_alpha=p["tukey_alpha"],
+                                   interp_type=p["interp_type"], accumulate=False, layout=layout, order=order,
+                                   output=output)
+        _lib.check(lib.mach_beamform(ctypes.byref(args), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
-    # the band kernel sums transmits in a different order: keep it to single-transmit requests so that
-    # a tuned layout never changes results (all general-kernel layouts are bit-identical)
-    best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1)
-    key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox)
+    # the band kernel sums transmits in a different order and has no epilogues: keep it to
+    # single-transmit signal requests, so that a tuned layout never changes results
+    best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1 and output == 0)
+    key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox, output)
     if record:
-        table().record(key, best, times, {"workload": acq.name, "storage": storage})
+        table().record(key, best, times, {"workload": acq.name, "storage": storage,
+                                          "output": ("signal", "power", "envelope")[output]})
     return key, best, times
 
 
@@ -178,16 +207,19 @@ def _main():
     ap.add_argument("--workloads", default="plane_wave,compound,doppler,volumetric")
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--power", default="doppler", help="workloads also tuned for the power Doppler output")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     for name in args.workloads.split(","):
         acq = phantoms.make(name, device=dev)
         storages = ("fp16",) if name == "volumetric" else ("c64", "fp16")
+        outputs = (0, 1) if name in args.power.split(",") else (0,)
         for st in storages:
-            key, best, times = tune_acquisition(acq, st, args.reps)
-            ranked = sorted(times.items(), key=lambda kv: kv[1])
-            print(f"{name:11s} {st:5s} {key}: best {hex(best)} {times[best]:.4f} ms; "
-                  + ", ".join(f"{hex(k)}={v:.4f}" for k, v in ranked[:6]), flush=True)
+            for output in outputs:
+                key, best, times = tune_acquisition(acq, st, args.reps, output=output)
+                ranked = sorted(times.items(), key=lambda kv: kv[1])
+                print(f"{name:11s} {st:5s} {key}: best {hex(best)} {times[best]:.4f} ms; "
+                      + ", ".join(f"{hex(k)}={v:.4f}" for k, v in ranked[:6]), flush=True)
         del acq
         torch.cuda.empty_cache()
     print("saved", table().save(args.out), flush=True)

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(256) void das_band_kernel(const DasParams p) {
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y,
+                                  p.xcd_group > 0 ? (uint32_t)p.xcd_group : 0u);
     const uint32_t tile = lb % gridDim.x, chunk = lb / gridDim.x;
     const int64_t f_base = (int64_t)chunk * FC;
     const int64_t f0 = f_base + (int64_t)lane * FPL;

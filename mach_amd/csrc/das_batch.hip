@@ -1,5 +1,6 @@
-// Batched delay-and-sum: many requests of different shapes, dtypes and interpolation types in one
-// persistent-kernel launch, replayed from a captured hipGraph.
+// Batched delay-and-sum, host side: many requests of different shapes, dtypes and interpolation types
+// in one submission of persistent kernels (das_batch_tc1.hip, das_batch_tcn.hip), replayed from a
+// captured hipGraph.
 //
 // A batch is a table of BatchItem descriptors (one per request), each carrying the request's
 // DasParams, its kernel variant and the size of its work grid (voxel groups x frame chunks, the
@@ -9,7 +10,7 @@
 // request spreads over every CU. Every workgroup exits once the counter passes the total.
 //
 // Host side: two slots, each with pinned staging, a device descriptor table, a counter and an
-// instantiated graph [H2D copy of the table -> counter reset kernel -> persistent kernel]. A slot is
+// instantiated graph [H2D copy of the table -> counter reset kernel -> persistent kernels]. A slot is
 // reused only after the event recorded behind its previous submission has completed, so the host
 // never rewrites staging that a pending copy still reads.
 #include <hip/hip_runtime.h>
@@ -19,122 +20,19 @@
 #include <new>
 
 #include "das_batch.h"
-#include "das_device.h"
+#include "das_batch_types.h"
 
 namespace mach {
 
+using namespace batch;
+
 namespace {
-
-using namespace dev;
-
-// (lanes per voxel, frames per lane) variants compiled into the batched kernels: every layout the
-// plain kernel's heuristic picks for ensembles (16x4, 32x2, 64x1) and for short frame counts, the
-// autotuner's winners for the benchmark workloads (32x2 and 16x4 ensembles, 2x4 volumetric, 4x1 single
-// frame, 1x1 compounding), and fallbacks for every frame count.
-constexpr int kBatchLayouts[][2] = {{16, 4}, {32, 2}, {64, 1}, {8, 4}, {4, 4}, {2, 4}, {1, 4},
-                                    {16, 2}, {4, 2}, {1, 2}, {16, 1}, {4, 1}, {1, 1}};
-constexpr int kNumBatchLayouts = sizeof(kBatchLayouts) / sizeof(kBatchLayouts[0]);
-
-// A batch has two sections, drained by two persistent kernels launched back to back: single-transmit
-// requests (section 0, no transmit chunking) and multi-transmit requests (section 1, chunks of
-// kTxChunk transmits). Each kernel's register budget, and so its occupancy, is set by its own
-// variants only: the single-transmit variants need about half the registers of the chunked ones.
-constexpr int kSections = 2;
-
-struct BatchHeader {
-    uint32_t n_items[kSections];
-    uint32_t total_blocks[kSections];
-    uint32_t first_item[kSections];
-    uint32_t pad[2];
-};
-
-struct BatchItem {
-    DasParams p;
-    int32_t store;
-    int32_t interp;
-    int32_t layout;  // index into kBatchLayouts
-    uint32_t blocks_x;
-    uint32_t blocks_y;
-    uint32_t first_block;  // within the item's section
-    uint32_t pad[2];
-};
-
-template <int ST, int INTERP, int TC>
-__device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by) {
-    switch (it.layout) {
-        case 0: das_block<ST, INTERP, 16, 4, TC>(it.p, bx, by); break;
-        case 1: das_block<ST, INTERP, 32, 2, TC>(it.p, bx, by); break;
-        case 2: das_block<ST, INTERP, 64, 1, TC>(it.p, bx, by); break;
-        case 3: das_block<ST, INTERP, 8, 4, TC>(it.p, bx, by); break;
-        case 4: das_block<ST, INTERP, 4, 4, TC>(it.p, bx, by); break;
-        case 5: das_block<ST, INTERP, 2, 4, TC>(it.p, bx, by); break;
-        case 6: das_block<ST, INTERP, 1, 4, TC>(it.p, bx, by); break;
-        case 7: das_block<ST, INTERP, 16, 2, TC>(it.p, bx, by); break;
-        case 8: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by); break;
-        case 9: das_block<ST, INTERP, 1, 2, TC>(it.p, bx, by); break;
-        case 10: das_block<ST, INTERP, 16, 1, TC>(it.p, bx, by); break;
-        case 11: das_block<ST, INTERP, 4, 1, TC>(it.p, bx, by); break;
-        default: das_block<ST, INTERP, 1, 1, TC>(it.p, bx, by); break;
-    }
-}
-
-template <int ST, int TC>
-__device__ __forceinline__ void run_interp(const BatchItem &it, uint32_t bx, uint32_t by) {
-    switch (it.interp) {
-        case 0: run_layout<ST, 0, TC>(it, bx, by); break;
-        case 2: run_layout<ST, 2, TC>(it, bx, by); break;
-        default: run_layout<ST, 1, TC>(it, bx, by); break;
-    }
-}
-
-__global__ void batch_reset_counter(uint32_t *counter) {
-    if (threadIdx.x < kSections) counter[threadIdx.x] = 0u;
-}
-
-// Persistent kernel for one section. TC is the transmit chunk the plain kernel uses for the same
-// requests (1 for single-transmit requests, kTxChunk otherwise), so a request's result is identical
-// in both paths.
-template <int TC>
-__global__ __launch_bounds__(256) void das_batch_kernel(const BatchHeader *__restrict__ header,
-                                                        const BatchItem *__restrict__ all_items,
-                                                        uint32_t *counters) {
-    constexpr int kSec = TC > 1 ? 1 : 0;
-    __shared__ uint32_t s_work;
-    const uint32_t n_items = header->n_items[kSec];
-    const uint32_t total = header->total_blocks[kSec];
-    const BatchItem *__restrict__ items = all_items + header->first_item[kSec];
-    uint32_t *counter = counters + kSec;
-    for (;;) {
-        __syncthreads();  // everyone has read the previous s_work
-        if (threadIdx.x == 0) s_work = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t w = s_work;
-        if (w >= total) break;  // block-uniform: the whole workgroup leaves together
-        uint32_t lo = 0, hi = n_items - 1;  // last item with first_block <= w (n_items >= 1 here)
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (items[mid].first_block <= w) lo = mid; else hi = mid - 1;
-        }
-        const BatchItem &it = items[lo];
-        const uint32_t local = w - it.first_block;
-        const uint32_t bx = local % it.blocks_x, by = local / it.blocks_x;
-        switch (it.store) {
-            case kStoreF32: run_interp<kStoreF32, TC>(it, bx, by); break;
-            case kStoreC32H: run_interp<kStoreC32H, TC>(it, bx, by); break;
-            default: run_interp<kStoreC64, TC>(it, bx, by); break;
-        }
-    }
-}
 
 int layout_index(DasLayout l) {
     for (int i = 0; i < kNumBatchLayouts; ++i)
         if (kBatchLayouts[i][0] == l.lanes_per_voxel && kBatchLayouts[i][1] == l.frames_per_lane) return i;
     return -1;
 }
-
-}  // namespace
-
-namespace {
 
 int ilog2(int v) {
     int r = 0;
@@ -150,6 +48,7 @@ int ilog2(int v) {
 // to more frames per lane. Band layouts run as the general kernel's heuristic choice.
 DasLayout batch_layout(const DasParams &p, int store, DasLayout want) {
     if (want.lanes_per_voxel == 0) want = choose_layout(p, store);
+    want.chunks_per_lane = 1;  // the batched kernels walk one frame group per lane
     if (layout_index(want) >= 0) return want;
     const int target = ilog2(want.lanes_per_voxel * want.frames_per_lane);
     int best = kNumBatchLayouts - 1, best_d = 1 << 30;
@@ -162,7 +61,7 @@ DasLayout batch_layout(const DasParams &p, int store, DasLayout want) {
             best_d = d;
         }
     }
-    return DasLayout{kBatchLayouts[best][0], kBatchLayouts[best][1]};
+    return DasLayout{kBatchLayouts[best][0], kBatchLayouts[best][1], 1};
 }
 
 bool batch_layout_supported(DasLayout l) { return layout_index(l) >= 0; }
@@ -194,16 +93,12 @@ size_t items_offset() { return (sizeof(BatchHeader) + 255) & ~size_t(255); }
 // replayed for every batch; an empty section's workgroups exit at once), only non-empty ones otherwise.
 hipError_t enqueue_direct(BatchState *b, BatchSlot &s, hipStream_t stream, unsigned sections) {
     hipError_t e = hipMemcpyAsync(s.d_buf, s.h_buf, b->table_bytes, hipMemcpyHostToDevice, stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(batch_reset_counter, dim3(1), dim3(64), 0, stream, s.d_counter);
+    if (e == hipSuccess) e = launch_reset_counters(stream, s.d_counter);
     const BatchHeader *hdr = (const BatchHeader *)s.d_buf;
     const BatchItem *items = (const BatchItem *)((char *)s.d_buf + items_offset());
-    if (sections & 1u)
-        hipLaunchKernelGGL(das_batch_kernel<1>, dim3(b->grid[0]), dim3(256), 0, stream, hdr, items, s.d_counter);
-    if (sections & 2u)
-        hipLaunchKernelGGL(das_batch_kernel<kTxChunk>, dim3(b->grid[1]), dim3(256), 0, stream, hdr, items,
-                           s.d_counter);
-    return hipGetLastError();
+    if (e == hipSuccess && (sections & 1u)) e = launch_section0((unsigned)b->grid[0], stream, hdr, items, s.d_counter);
+    if (e == hipSuccess && (sections & 2u)) e = launch_section1((unsigned)b->grid[1], stream, hdr, items, s.d_counter);
+    return e;
 }
 
 hipError_t capture(BatchState *b, BatchSlot &s) {
@@ -233,9 +128,8 @@ hipError_t batch_create(uint32_t max_items, bool use_graph, BatchState **out) {
     int dev = 0, cus = 0, per_cu[kSections] = {0, 0};
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], das_batch_kernel<1>, 256, 0);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], das_batch_kernel<kTxChunk>, 256, 0);
+    if (e == hipSuccess) e = section_occupancy(0, &per_cu[0]);
+    if (e == hipSuccess) e = section_occupancy(1, &per_cu[1]);
     for (int i = 0; i < kSections; ++i) b->grid[i] = (per_cu[i] > 0 ? per_cu[i] : 1) * (cus > 0 ? cus : 1);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) {
         BatchSlot &s = b->slot[i];

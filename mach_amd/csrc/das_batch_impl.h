@@ -1,0 +1,81 @@
+// Device code of the persistent batched kernels, included by das_batch_tc1.hip and das_batch_tcn.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "das_batch_types.h"
+#include "das_device.h"
+
+namespace mach {
+namespace batch {
+namespace {
+
+using namespace dev;
+
+template <int ST, int INTERP, int TC>
+__device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by) {
+    switch (it.layout) {
+        case 0: das_block<ST, INTERP, 16, 4, TC>(it.p, bx, by); break;
+        case 1: das_block<ST, INTERP, 32, 2, TC>(it.p, bx, by); break;
+        case 2: das_block<ST, INTERP, 64, 1, TC>(it.p, bx, by); break;
+        case 3: das_block<ST, INTERP, 8, 4, TC>(it.p, bx, by); break;
+        case 4: das_block<ST, INTERP, 4, 4, TC>(it.p, bx, by); break;
+        case 5: das_block<ST, INTERP, 2, 4, TC>(it.p, bx, by); break;
+        case 6: das_block<ST, INTERP, 1, 4, TC>(it.p, bx, by); break;
+        case 7: das_block<ST, INTERP, 16, 2, TC>(it.p, bx, by); break;
+        case 8: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by); break;
+        case 9: das_block<ST, INTERP, 1, 2, TC>(it.p, bx, by); break;
+        case 10: das_block<ST, INTERP, 16, 1, TC>(it.p, bx, by); break;
+        case 11: das_block<ST, INTERP, 4, 1, TC>(it.p, bx, by); break;
+        default: das_block<ST, INTERP, 1, 1, TC>(it.p, bx, by); break;
+    }
+}
+
+template <int ST, int TC>
+__device__ __forceinline__ void run_interp(const BatchItem &it, uint32_t bx, uint32_t by) {
+    switch (it.interp) {
+        case 0: run_layout<ST, 0, TC>(it, bx, by); break;
+        case 2: run_layout<ST, 2, TC>(it, bx, by); break;
+        default: run_layout<ST, 1, TC>(it, bx, by); break;
+    }
+}
+
+// Persistent kernel for one section. TC is the transmit chunk the plain kernel uses for the same
+// requests (1 for single-transmit requests, kTxChunk otherwise), so a request's result is identical
+// in both paths.
+template <int TC>
+__global__ __launch_bounds__(256) void das_batch_kernel(const BatchHeader *__restrict__ header,
+                                                        const BatchItem *__restrict__ all_items,
+                                                        uint32_t *counters) {
+    constexpr int kSec = TC > 1 ? 1 : 0;
+    __shared__ uint32_t s_work;
+    const uint32_t n_items = header->n_items[kSec];
+    const uint32_t total = header->total_blocks[kSec];
+    const BatchItem *__restrict__ items = all_items + header->first_item[kSec];
+    uint32_t *counter = counters + kSec;
+    for (;;) {
+        __syncthreads();  // everyone has read the previous s_work
+        if (threadIdx.x == 0) s_work = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = s_work;
+        if (w >= total) break;  // block-uniform: the whole workgroup leaves together
+        uint32_t lo = 0, hi = n_items - 1;  // last item with first_block <= w (n_items >= 1 here)
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (items[mid].first_block <= w) lo = mid; else hi = mid - 1;
+        }
+        const BatchItem &it = items[lo];
+        const uint32_t local = w - it.first_block;
+        const uint32_t bx = local % it.blocks_x, by = local / it.blocks_x;
+        switch (it.store) {
+            case kStoreF32: run_interp<kStoreF32, TC>(it, bx, by); break;
+            case kStoreC32H: run_interp<kStoreC32H, TC>(it, bx, by); break;
+            default: run_interp<kStoreC64, TC>(it, bx, by); break;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace batch
+}  // namespace mach

@@ -227,24 +227,64 @@ __device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, i
     }
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (linear id i runs on XCD i % 8). Remap so
-// that each XCD walks one contiguous range of logical blocks: neighbouring voxels (which read the same
-// channel-data rows) then share an L2.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id i runs on XCD i % 8), each XCD with
+// its own L2. Two remaps of linear ids to logical blocks:
+//   group == 0: each XCD walks one contiguous eighth of the logical blocks, so neighbouring voxels
+//               (which read the same channel-data rows) share an L2. With a depth-major voxel order
+//               the eighths differ in depth, and work grows with depth (the f-number aperture widens),
+//               so the deepest XCD finishes last.
+//   group  > 0: logical blocks in runs of `group`, run r on XCD r % 8. All XCDs sweep the depth range
+//               together (balanced), each taking every eighth run; consecutive runs of one XCD are
+//               8 * group blocks apart, close in depth, so their sample windows still overlap in L2.
+// Both are bijections on [0, n): ids in the last incomplete set of 8 * group blocks map to themselves.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n, uint32_t group) {
     constexpr uint32_t kXcd = 8;
-    const uint32_t per = n / kXcd, rem = n % kXcd;
     const uint32_t xcd = bid % kXcd, local = bid / kXcd;
-    return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    if (group == 0) {
+        const uint32_t per = n / kXcd, rem = n % kXcd;
+        return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    }
+    const uint32_t full = n / (kXcd * group) * (kXcd * group);
+    if (bid >= full) return bid;
+    return ((local / group) * kXcd + xcd) * group + local % group;
+}
+
+// Maximum of `x` over the wave (every lane gets it).
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = fmaxf(x, __shfl_xor(x, m));
+    return x;
+}
+
+// Raises *peak to `x` if larger. Non-negative float32 values order like their bit patterns, so an
+// unsigned atomic max gives the maximum whatever the order of the updates.
+__device__ __forceinline__ void peak_max(float *peak, float x) {
+    atomicMax((unsigned int *)peak, __float_as_uint(x));
 }
 
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
-// order) and frame chunk `by` (LF * FPL frames). Used by the plain kernel and the batched kernel.
+// order) and frame block `by`. Used by the plain kernel and the batched kernel.
+//
+// Frames: a lane owns CH groups of FPL consecutive frames, group c starting at frame
+// (by * CH + c) * (LF * FPL) + lf * FPL; a block covers LF * FPL * CH frames. The per-pair setup, its
+// broadcast and the tap offsets are computed once for all CH groups.
 // Transmits are taken TC at a time: the transmit-independent pair geometry (aperture, receive delay,
-// apodisation) is computed once per chunk and reused for its TC transmits. The per-voxel summation
-// order is (transmit chunk, element, transmit within chunk) whatever the layout.
-template <int ST, int INTERP, int LF, int FPL, int TC = 1>
+// apodisation) is computed once per chunk and reused for its TC transmits. The per-voxel, per-frame
+// summation order is (transmit chunk, element, transmit within chunk) whatever the layout.
+//
+// Epilogues (p.output, uniform over the launch):
+//   kOutSignal   - out[v][f] (+)= y, the reference's output.
+//   kOutEnvelope - out[v][f] = |y| (float32).
+//   kOutPower    - out[v] (+)= sum_f |y|^2 (float32). The block walks every frame block itself (the
+//                  launch has one block row), each lane summing its frames in frame-group order; the LF
+//                  lanes of a voxel are then combined by an xor butterfly, so the result is
+//                  deterministic (and the same for every CH).
+// With p.peak set, the maximum value written is folded into *peak.
+template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1>
 __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
     constexpr int LV = 64 / LF;
+    constexpr int kGroup = LF * FPL;  // frames per group across the LF lanes of a voxel
+    constexpr bool kComplex = ST != kStoreF32;
     // The empty asm makes the thread index opaque, so nothing derived from it is hoisted out of a
     // caller's loop. In the persistent batched kernel (which runs every layout variant inside one
     // loop) hoisting kept the lane constants of all variants live at once: 144 instead of 122 VGPRs.
@@ -258,8 +298,6 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
     const int64_t slot = vgroup * LV + lv;
     const bool vok = slot < p.n_vox;
     const int64_t v = (vok && p.order != nullptr) ? (int64_t)p.order[slot] : slot;
-    const int64_t f0 = (int64_t)by * (LF * FPL) + (int64_t)lf * FPL;
-    const bool fok = f0 < p.n_frames;
 
     double xv = 0.0, yv = 0.0, zv = 1.0;
     if (vok) {
@@ -271,76 +309,134 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
     const double R2 = R * R;
     const double inv_R = 1.0 / R;
 
-    float acc_re[FPL], acc_im[FPL];
+    const bool power = p.output == kOutPower;
+    const int64_t n_blocks_f = power ? (p.n_frames + kGroup * CH - 1) / (kGroup * CH) : 1;
+    float pw = 0.f, vmax = 0.f;
+    for (int64_t fb = 0; fb < n_blocks_f; ++fb) {
+        const int64_t fblock = power ? fb : (int64_t)by;
+        const int64_t f0 = fblock * (kGroup * CH) + (int64_t)lf * FPL;  // first frame of group 0
+        bool fok[CH];
 #pragma unroll
-    for (int k = 0; k < FPL; ++k) acc_re[k] = acc_im[k] = 0.f;
+        for (int c = 0; c < CH; ++c) fok[c] = f0 + (int64_t)c * kGroup < p.n_frames;
 
-    for (int64_t t0 = 0; t0 < p.n_tx; t0 += TC) {
-        const int nt = (int)((p.n_tx - t0) < TC ? (p.n_tx - t0) : TC);
-        double Tv[TC];
+        float acc_re[CH][FPL], acc_im[CH][FPL];
 #pragma unroll
-        for (int tt = 0; tt < TC; ++tt) Tv[tt] = (vok && tt < nt) ? (double)p.arrivals[(t0 + tt) * p.n_vox + v] : 0.0;
-        for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
-            const int64_t e = e0 + lf;
-            PairSetup s[TC];
-            PairGeom g{0.0, 0.f};
-            if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+        for (int c = 0; c < CH; ++c)
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt) {
-                s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
-                if (tt < nt) s[tt] = pair_time<INTERP>(p, g, Tv[tt]);
-            }
-            // Only elements that some voxel of this wave uses under some transmit of the chunk: fold
-            // the wave's ballot (bit lv * LF + j: lane lv * LF + j computed element e0 + j of voxel lv)
-            // onto the LF element slots and walk the set bits with a scalar loop. The predicate is the
-            // pair's validity only: the lane that computed a pair may itself own no frames.
-            uint64_t jm = 0ull;
+            for (int k = 0; k < FPL; ++k) acc_re[c][k] = acc_im[c][k] = 0.f;
+
+        for (int64_t t0 = 0; t0 < p.n_tx; t0 += TC) {
+            const int nt = (int)((p.n_tx - t0) < TC ? (p.n_tx - t0) : TC);
+            double Tv[TC];
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt) jm |= __ballot(s[tt].i0 >= 0);
-            if constexpr (LF < 64) {
-#pragma unroll
-                for (int sh = LF; sh < 64; sh <<= 1) jm |= jm >> sh;
-                jm &= (1ull << LF) - 1ull;
-            }
-            // Summation order per voxel: (transmit chunk, element, transmit) -- the same for every
-            // layout, so all layouts (and the batched kernel) give bit-identical results.
-            while (jm != 0ull) {
-                const int j = __builtin_ctzll(jm);
-                jm &= jm - 1ull;
-                const int64_t e_abs = e0 + j;
+            for (int tt = 0; tt < TC; ++tt)
+                Tv[tt] = (vok && tt < nt) ? (double)p.arrivals[(t0 + tt) * p.n_vox + v] : 0.0;
+            for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
+                const int64_t e = e0 + lf;
+                PairSetup s[TC];
+                PairGeom g{0.0, 0.f};
+                if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
 #pragma unroll
                 for (int tt = 0; tt < TC; ++tt) {
-                    if (tt >= nt) break;
-                    const int i0 = bcast<LF>(s[tt].i0, j, lane);
-                    const float frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
-                    const float wc = __int_as_float(bcast<LF>(__float_as_int(s[tt].wc), j, lane));
-                    const float ws = __int_as_float(bcast<LF>(__float_as_int(s[tt].ws), j, lane));
-                    if (i0 >= 0 && fok) {
-                        const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;  // uniform
-                        const int64_t off = row0 + f0 +
-                                            (p.rows32 ? (int64_t)((uint32_t)i0 * (uint32_t)p.n_frames)
-                                                      : (int64_t)i0 * p.n_frames);
-                        accumulate_pair<ST, INTERP, FPL>(p.data, off, p.tap_step, frac, wc, ws, acc_re, acc_im);
+                    s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
+                    if (tt < nt) s[tt] = pair_time<INTERP>(p, g, Tv[tt]);
+                }
+                // Only elements that some voxel of this wave uses under some transmit of the chunk:
+                // fold the wave's ballot (bit lv * LF + j: lane lv * LF + j computed element e0 + j of
+                // voxel lv) onto the LF element slots and walk the set bits with a scalar loop. The
+                // predicate is the pair's validity only: the lane that computed a pair may itself own
+                // no frames.
+                uint64_t jm = 0ull;
+#pragma unroll
+                for (int tt = 0; tt < TC; ++tt) jm |= __ballot(s[tt].i0 >= 0);
+                if constexpr (LF < 64) {
+#pragma unroll
+                    for (int sh = LF; sh < 64; sh <<= 1) jm |= jm >> sh;
+                    jm &= (1ull << LF) - 1ull;
+                }
+                // Summation order per voxel and frame: (transmit chunk, element, transmit) -- the same
+                // for every layout, so all layouts (and the batched kernel) give bit-identical results.
+                while (jm != 0ull) {
+                    const int j = __builtin_ctzll(jm);
+                    jm &= jm - 1ull;
+                    const int64_t e_abs = e0 + j;
+#pragma unroll
+                    for (int tt = 0; tt < TC; ++tt) {
+                        if (tt >= nt) break;
+                        const int i0 = bcast<LF>(s[tt].i0, j, lane);
+                        const float frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
+                        const float wc = __int_as_float(bcast<LF>(__float_as_int(s[tt].wc), j, lane));
+                        const float ws = __int_as_float(bcast<LF>(__float_as_int(s[tt].ws), j, lane));
+                        if (i0 >= 0 && fok[0]) {
+                            const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;
+                            const int64_t off = row0 + f0 +
+                                                (p.rows32 ? (int64_t)((uint32_t)i0 * (uint32_t)p.n_frames)
+                                                          : (int64_t)i0 * p.n_frames);
+#pragma unroll
+                            for (int c = 0; c < CH; ++c) {
+                                if (c == 0 || fok[c])
+                                    accumulate_pair<ST, INTERP, FPL>(p.data, off + (int64_t)c * kGroup, p.tap_step,
+                                                                     frac, wc, ws, acc_re[c], acc_im[c]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int64_t fc = f0 + (int64_t)c * kGroup;
+            if (power) {
+                if (vok && fok[c]) {
+#pragma unroll
+                    for (int k = 0; k < FPL; ++k) {
+                        pw = fmaf(acc_re[c][k], acc_re[c][k], pw);
+                        if constexpr (kComplex) pw = fmaf(acc_im[c][k], acc_im[c][k], pw);
+                    }
+                }
+            } else if (p.output == kOutEnvelope) {
+                if (vok && fok[c]) {
+                    float *q = (float *)p.out + v * p.n_frames + fc;
+#pragma unroll
+                    for (int k = 0; k < FPL; ++k) {
+                        const float a = kComplex ? sqrtf(fmaf(acc_re[c][k], acc_re[c][k], acc_im[c][k] * acc_im[c][k]))
+                                                 : fabsf(acc_re[c][k]);
+                        q[k] = a;
+                        vmax = fmaxf(vmax, a);
+                    }
+                }
+            } else if (vok && fok[c]) {
+                const int64_t o = v * p.n_frames + fc;
+                if constexpr (ST == kStoreF32) {
+                    float *q = (float *)p.out + o;
+#pragma unroll
+                    for (int k = 0; k < FPL; ++k) q[k] = p.accumulate ? q[k] + acc_re[c][k] : acc_re[c][k];
+                } else {
+                    float *q = (float *)p.out + 2 * o;
+#pragma unroll
+                    for (int k = 0; k < FPL; ++k) {
+                        q[2 * k + 0] = p.accumulate ? q[2 * k + 0] + acc_re[c][k] : acc_re[c][k];
+                        q[2 * k + 1] = p.accumulate ? q[2 * k + 1] + acc_im[c][k] : acc_im[c][k];
                     }
                 }
             }
         }
     }
 
-    if (vok && fok) {
-        const int64_t o = v * p.n_frames + f0;
-        if constexpr (ST == kStoreF32) {
-            float *q = (float *)p.out + o;
+    if (power) {
 #pragma unroll
-            for (int k = 0; k < FPL; ++k) q[k] = p.accumulate ? q[k] + acc_re[k] : acc_re[k];
-        } else {
-            float *q = (float *)p.out + 2 * o;
-#pragma unroll
-            for (int k = 0; k < FPL; ++k) {
-                q[2 * k + 0] = p.accumulate ? q[2 * k + 0] + acc_re[k] : acc_re[k];
-                q[2 * k + 1] = p.accumulate ? q[2 * k + 1] + acc_im[k] : acc_im[k];
-            }
+        for (int m = 1; m < LF; m <<= 1) pw += __shfl_xor(pw, m);  // lanes lv*LF .. lv*LF+LF-1
+        if (vok && lf == 0) {
+            float *q = (float *)p.out + v;
+            const float r = p.accumulate ? *q + pw : pw;
+            *q = r;
+            vmax = r;
         }
+    }
+    if (p.output != kOutSignal && p.peak != nullptr) {
+        const float m = wave_max(vmax);
+        if (lane == 0 && m > 0.f) peak_max(p.peak, m);
     }
 }
 

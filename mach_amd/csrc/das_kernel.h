@@ -15,6 +15,10 @@ constexpr int kTxChunk = 8;
 
 // Everything a launch needs, already validated and converted. Geometry scalars are float64 values
 // of the float32 API scalars.
+// What the kernel writes (mach_output_kind): the beamformed signal [n_vox][n_frames], the power
+// Doppler image sum_f |y|^2 [n_vox] (float32), or the envelope |y| [n_vox][n_frames] (float32).
+enum OutputKind { kOutSignal = 0, kOutPower = 1, kOutEnvelope = 2 };
+
 struct DasParams {
     const void *data;        // [n_tx][n_rx][n_samples][n_frames] of the storage type
     const float *rx;         // [n_rx][3]
@@ -35,14 +39,41 @@ struct DasParams {
     int32_t accumulate;      // out += result (1) or out = result (0)
     int32_t apod;            // alpha > 0
     int32_t rows32;          // n_samples * n_frames < 2^32: sample offsets within a channel fit 32 bits
+    int32_t output;          // OutputKind
+    int32_t xcd_group;       // block-to-XCD remap (xcd_remap): 0 = contiguous eighths, > 0 = runs of this
+                             // many blocks; < 0 = chosen per launch (xcd_group_for)
+    float *peak;             // optional: maximum of the written POWER / ENVELOPE values (zeroed first)
 };
 
 // Frame-lane layout of a launch: LF lanes of a wave share one voxel and split its frames, each lane
 // handling FPL consecutive frames; a wave therefore covers 64 / LF voxels.
 struct DasLayout {
-    int lanes_per_voxel;   // LF in {1, 2, 4, 8, 16, 32, 64}
-    int frames_per_lane;   // FPL in {1, 2, 4}
+    int lanes_per_voxel;      // LF in {1, 2, 4, 8, 16, 32, 64}
+    int frames_per_lane;      // FPL in {1, 2, 4}
+    int chunks_per_lane = 1;  // CH in {1, 2, 4}: frame groups of LF * FPL frames a lane walks per setup
 };
+
+// (LF, FPL) layouts compiled with CH > 1 (frame ensembles, where the per-pair setup is worth sharing).
+constexpr bool multi_group_layout(int lf, int fpl) {
+    return (lf == 16 && fpl == 4) || (lf == 32 && fpl == 2) || (lf == 64 && fpl == 1) || (lf == 8 && fpl == 4);
+}
+
+// Layout code of the C ABI: (log2(CH) << 12) | (LF << 4) | FPL; LF == 0 selects the band kernel.
+inline DasLayout decode_layout(int32_t code) {
+    const int ch_log2 = (code >> 12) & 0xf;
+    return DasLayout{(code >> 4) & 0xff, code & 0xf, ch_log2 <= 2 ? 1 << ch_log2 : 0};
+}
+
+// Block-to-XCD remap of a plain launch with `blocks_y` frame-block rows (xcd_remap). Measured on gfx950
+// with depth-major voxel order: a launch with one block row and few transmits is limited by the
+// deepest XCD (work grows with depth), so interleaving runs of 8 blocks balances it -- volumetric
+// 50.5 -> 39.1 ms, Doppler power image 0.73 -> 0.49 ms, single plane wave 0.057 -> 0.054 ms --
+// while launches with several block rows (Doppler signal output: 0.558 vs 0.576 ms) or many
+// transmits (75-angle compounding: 5.40 vs 5.61 ms) keep more L2 reuse with contiguous eighths.
+inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_y) {
+    if (p.xcd_group >= 0) return (uint32_t)p.xcd_group;
+    return (blocks_y == 1 && p.n_tx < kTxChunk) ? 8u : 0u;
+}
 
 // Picks the layout for a problem (frame count and pointer alignment decide vector widths).
 DasLayout choose_layout(const DasParams &p, int store);

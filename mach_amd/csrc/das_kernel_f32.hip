@@ -1,0 +1,10 @@
+// Plain delay-and-sum kernels for float32 RF channel data.
+#include "das_kernel_impl.h"
+
+namespace mach {
+
+hipError_t launch_das_f32(const DasParams &p, int interp, DasLayout l, hipStream_t stream) {
+    return launch_das_store<kStoreF32>(p, interp, l, stream);
+}
+
+}  // namespace mach

@@ -1,0 +1,95 @@
+// Plain (one request per launch) delay-and-sum kernel templates and their launch dispatch, included by
+// one translation unit per storage type (das_kernel_f32.hip, das_kernel_c64.hip, das_kernel_c32h.hip)
+// so the variants compile in parallel. The device code is in das_device.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "das_device.h"
+#include "das_kernel.h"
+
+namespace mach {
+namespace {
+
+using namespace dev;
+
+template <int ST, int INTERP, int LF, int FPL, int TC, int CH>
+__global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
+    const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
+    das_block<ST, INTERP, LF, FPL, TC, CH>(p, lb % gridDim.x, lb / gridDim.x);
+}
+
+template <int ST, int INTERP, int LF, int FPL, int CH>
+hipError_t launch_one(const DasParams &p, hipStream_t stream) {
+    constexpr int LV = 64 / LF;
+    const int64_t n_vgroups = (p.n_vox + LV - 1) / LV;
+    const int64_t blocks_x = (n_vgroups + 3) / 4;
+    // the power epilogue walks all frame blocks inside each block
+    const int64_t per = (int64_t)LF * FPL * CH;
+    const int64_t blocks_y = p.output == kOutPower ? 1 : (p.n_frames + per - 1) / per;
+    if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks_x, (unsigned)blocks_y, 1);
+    DasParams q = p;
+    q.xcd_group = (int32_t)xcd_group_for(p, blocks_y);
+    if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH>), grid, dim3(256, 1, 1), 0, stream, q);
+    } else {
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, 1, CH>), grid, dim3(256, 1, 1), 0, stream, q);
+    }
+    return hipGetLastError();
+}
+
+// Several frame groups per lane (CH > 1) are compiled for the ensemble layouts only
+// (multi_group_layout).
+template <int ST, int INTERP, int LF, int FPL>
+hipError_t dispatch_ch(const DasParams &p, int ch, hipStream_t stream) {
+    if constexpr (multi_group_layout(LF, FPL)) {
+        if (ch == 4) return launch_one<ST, INTERP, LF, FPL, 4>(p, stream);
+        if (ch == 2) return launch_one<ST, INTERP, LF, FPL, 2>(p, stream);
+    }
+    if (ch != 1) return hipErrorInvalidValue;
+    return launch_one<ST, INTERP, LF, FPL, 1>(p, stream);
+}
+
+template <int ST, int INTERP, int LF>
+hipError_t dispatch_fpl(const DasParams &p, int fpl, int ch, hipStream_t stream) {
+    switch (fpl) {
+        case 4: return dispatch_ch<ST, INTERP, LF, 4>(p, ch, stream);
+        case 2: return dispatch_ch<ST, INTERP, LF, 2>(p, ch, stream);
+        default: return dispatch_ch<ST, INTERP, LF, 1>(p, ch, stream);
+    }
+}
+
+template <int ST, int INTERP>
+hipError_t dispatch_lf(const DasParams &p, DasLayout l, hipStream_t stream) {
+    const int k = l.frames_per_lane, ch = l.chunks_per_lane;
+    switch (l.lanes_per_voxel) {
+        case 64: return dispatch_fpl<ST, INTERP, 64>(p, k, ch, stream);
+        case 32: return dispatch_fpl<ST, INTERP, 32>(p, k, ch, stream);
+        case 16: return dispatch_fpl<ST, INTERP, 16>(p, k, ch, stream);
+        case 8: return dispatch_fpl<ST, INTERP, 8>(p, k, ch, stream);
+        case 4: return dispatch_fpl<ST, INTERP, 4>(p, k, ch, stream);
+        case 2: return dispatch_fpl<ST, INTERP, 2>(p, k, ch, stream);
+        default: return dispatch_fpl<ST, INTERP, 1>(p, k, ch, stream);
+    }
+}
+
+template <int ST>
+hipError_t dispatch_interp(const DasParams &p, int interp, DasLayout l, hipStream_t stream) {
+    switch (interp) {
+        case kNearest: return dispatch_lf<ST, kNearest>(p, l, stream);
+        case kQuadratic: return dispatch_lf<ST, kQuadratic>(p, l, stream);
+        default: return dispatch_lf<ST, kLinear>(p, l, stream);
+    }
+}
+
+}  // namespace
+
+// Entry of one storage type's translation unit.
+template <int ST>
+hipError_t launch_das_store(const DasParams &p, int interp, DasLayout l, hipStream_t stream) {
+    return dispatch_interp<ST>(p, interp, l, stream);
+}
+
+}  // namespace mach

@@ -1,0 +1,53 @@
+// Post-processing kernels of the epilogue outputs: logarithmic compression to dB and the small fills
+// the epilogues need (graph-safe kernels rather than hipMemsetAsync).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "das_post.h"
+
+namespace mach {
+namespace {
+
+// float64 machine epsilon, the value the reference substitutes for exact zeros before taking the
+// logarithm (np.finfo(float).eps in src/mach/_vis.py:25), rounded to float32.
+constexpr float kZeroFloor = 2.220446049250313e-16f;
+
+__device__ __forceinline__ float db_of(float x, float k) {
+    const float a = (x == 0.f) ? kZeroFloor : fabsf(x);
+    return k * log10f(a);
+}
+
+// One thread per element, grid-stride so that any n fits the launch.
+__global__ __launch_bounds__(256) void to_db_kernel(float *__restrict__ x, uint64_t n, const float *__restrict__ peak,
+                                                    float k) {
+    const float ref = db_of(*peak, k);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = db_of(x[i], k) - ref;
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float *__restrict__ x, uint64_t n, float value) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = value;
+}
+
+unsigned grid_for(uint64_t n) {
+    const uint64_t blocks = (n + 255) / 256;
+    return (unsigned)(blocks < 8192 ? (blocks > 0 ? blocks : 1) : 8192);
+}
+
+}  // namespace
+
+hipError_t launch_to_db(float *x, uint64_t n, const float *peak, bool power_mode, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(to_db_kernel, dim3(grid_for(n)), dim3(256), 0, stream, x, n, peak, power_mode ? 10.f : 20.f);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *x, uint64_t n, float value, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, stream, x, n, value);
+    return hipGetLastError();
+}
+
+}  // namespace mach

@@ -6,6 +6,7 @@
 #include <stdio.h>
 
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -15,6 +16,7 @@
 #include "das_band.h"
 #include "das_batch.h"
 #include "das_kernel.h"
+#include "das_post.h"
 
 struct mach_batch {
     mach::BatchState *state;
@@ -32,6 +34,19 @@ int fail(int status, const std::string &msg) {
 int hip_fail(hipError_t e, const char *what) {
     return fail(MACH_ERR_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorName(e) + " (" +
                                   hipGetErrorString(e) + ")");
+}
+
+// Block-to-XCD remap granularity (xcd_remap): MACH_AMD_XCD_GROUP=<n> forces contiguous eighths (0) or
+// runs of n blocks for every launch, for measurements; unset, each launch picks (mach::xcd_group_for).
+// Read once per process.
+int32_t xcd_group_override() {
+    static const int32_t g = [] {
+        const char *env = std::getenv("MACH_AMD_XCD_GROUP");
+        if (env == nullptr || *env == '\0') return (int32_t)-1;
+        const long v = std::strtol(env, nullptr, 10);
+        return (int32_t)(v < 0 ? 0 : (v > 4096 ? 4096 : v));
+    }();
+    return g;
 }
 
 int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
@@ -54,14 +69,21 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     const long double total = (long double)a->n_tx * a->n_rx * a->n_samples * a->n_frames;
     if (total > (long double)INT64_MAX / 4)
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: channel data size exceeds the 64-bit indexing limit");
-    const bool need_data = a->n_vox > 0 && a->n_frames > 0;
+    if (a->output < MACH_OUTPUT_SIGNAL || a->output > MACH_OUTPUT_ENVELOPE)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: unsupported output kind " + std::to_string(a->output));
+    if (a->output == MACH_OUTPUT_ENVELOPE && a->accumulate)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: the envelope output cannot be accumulated (|a + b| != |a| + |b|)");
+    if (a->output == MACH_OUTPUT_SIGNAL && a->peak != nullptr)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: peak applies to the power and envelope outputs only");
+    // the power output has n_vox values even when there are no frames (a sum over nothing is 0)
+    const bool need_data = a->n_vox > 0 && (a->n_frames > 0 || a->output == MACH_OUTPUT_POWER);
     if (need_data && a->out == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: out is NULL");
     if (need_data && a->scan_coords_m == nullptr)
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: scan_coords_m is NULL");
     const bool need_inputs = need_data && a->n_tx > 0 && a->n_rx > 0;
     if (need_inputs && (a->rx_coords_m == nullptr || a->tx_wave_arrivals_s == nullptr))
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: rx_coords_m / tx_wave_arrivals_s is NULL");
-    if (need_inputs && a->n_samples > 0 && a->channel_data == nullptr)
+    if (need_inputs && a->n_samples > 0 && a->n_frames > 0 && a->channel_data == nullptr)
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: channel_data is NULL");
 
     p->data = a->channel_data;
@@ -87,6 +109,9 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->accumulate = a->accumulate ? 1 : 0;
     p->apod = a->tukey_alpha > 0.0f ? 1 : 0;  // kernel.cu:749
     p->rows32 = (a->n_samples * a->n_frames) < (1ull << 32) ? 1 : 0;
+    p->output = a->output;
+    p->peak = a->peak;
+    p->xcd_group = xcd_group_override();
     return MACH_OK;
 }
 
@@ -111,9 +136,16 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
         out->layout = mach::choose_layout(p, a->data_type);
         return MACH_OK;
     }
-    const mach::DasLayout l{(a->layout >> 4) & 0xfff, a->layout & 0xf};
+    const mach::DasLayout l = mach::decode_layout(a->layout);
+    if (l.lanes_per_voxel == 0 && p.output != mach::kOutSignal) {  // epilogues live in the general kernel
+        out->layout = mach::choose_layout(p, a->data_type);
+        return MACH_OK;
+    }
     if (l.lanes_per_voxel == 0) {
         const int k = mach::band_frames_per_lane(p, a->data_type);
+        if (l.chunks_per_lane != 1)
+            return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
+                                                       " (band kernel) takes no frame-group count");
         if (k == 0 || (l.frames_per_lane != 1 && l.frames_per_lane != 2 && l.frames_per_lane != 4) ||
             p.n_frames % (64 * l.frames_per_lane) != 0 || l.frames_per_lane > k)
             return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
@@ -127,7 +159,8 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: layout " + std::to_string(a->layout) +
                                                    " is not valid for this request (lanes_per_voxel must be a power of"
                                                    " two <= 64, frames_per_lane in {1, 2, 4} dividing n_frames and"
-                                                   " keeping vector accesses aligned)");
+                                                   " keeping vector accesses aligned, chunks_per_lane in {2, 4} only"
+                                                   " for 16x4, 32x2, 64x1 and 8x4)");
     out->layout = l;
     return MACH_OK;
 }
@@ -137,7 +170,19 @@ int mach_beamform(const mach_beamform_args *args, void *stream) {
     mach::DasParams p;
     int st = validate_and_build(args, &p);
     if (st != MACH_OK) return st;
-    if (p.n_vox == 0 || p.n_frames == 0) return MACH_OK;
+    if (p.n_vox > 0 && p.n_frames == 0 && p.output == mach::kOutPower) {  // sum over no frames
+        hipError_t e = hipSuccess;
+        if (!p.accumulate) e = mach::launch_fill((float *)p.out, (uint64_t)p.n_vox, 0.f, (hipStream_t)stream);
+        if (e == hipSuccess && p.peak != nullptr) e = mach::launch_fill(p.peak, 1, 0.f, (hipStream_t)stream);
+        return e == hipSuccess ? MACH_OK : hip_fail(e, "beamform power fill");
+    }
+    if (p.n_vox == 0 || p.n_frames == 0) {
+        if (p.peak != nullptr) {
+            const hipError_t e = mach::launch_fill(p.peak, 1, 0.f, (hipStream_t)stream);
+            if (e != hipSuccess) return hip_fail(e, "beamform peak fill");
+        }
+        return MACH_OK;
+    }
     Plan plan;
     st = resolve_plan(args, p, &plan);
     if (st != MACH_OK) return st;
@@ -174,6 +219,8 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
     std::vector<mach::DasLayout> layouts(n_requests);
     for (uint32_t i = 0; i < n_requests; ++i) {
         int st = validate_and_build(&requests[i], &params[i]);
+        if (st == MACH_OK && params[i].output != mach::kOutSignal)
+            st = fail(MACH_ERR_UNSUPPORTED, "Error: batches run the signal output only");
         Plan plan{false, 0, mach::DasLayout{1, 1}};
         if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0) st = resolve_plan(&requests[i], params[i], &plan);
         if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
@@ -243,6 +290,15 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
     return beamform_single(MACH_DATA_F32, channel_data, rx_coords_m, scan_coords_m, tx_wave_arrivals_s, out, n_rx,
                            n_samples, n_frames, n_vox, f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s,
                            modulation_freq_hz, tukey_alpha, interp_type, stream);
+}
+
+int mach_to_db(float *x, uint64_t n, const float *peak, int32_t power_mode, void *stream) {
+    g_last_error.clear();
+    if (n == 0) return MACH_OK;
+    if (x == nullptr || peak == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: x / peak is NULL");
+    const hipError_t e = mach::launch_to_db(x, n, peak, power_mode != 0, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mach_to_db");
+    return MACH_OK;
 }
 
 const char *mach_last_error(void) { return g_last_error.c_str(); }

@@ -27,6 +27,14 @@ What it restates (reference file:line, all under ``src/mach/``):
 * accumulation: out[v, f] += sum over transmits and elements (``kernel.cu:674-680``); compounding is
   the sum over transmits (``experimental.py:80-97``).
 
+Post-processing the reference leaves to its callers, restated for the kernel's epilogues:
+
+* power Doppler: sum over frames of |y|^2, ``np.square(np.abs(result)).sum(axis=-1)``
+  (``examples/doppler.py:252``; also ``tests/compare/test_pymust.py:317-318``) -> ``power_doppler64``.
+* envelope |y| and its dB image: ``db`` replaces exact zeros by ``np.finfo(float).eps`` and takes
+  10 log10 (power) or 20 log10 (amplitude) of the magnitude; ``db_zero`` subtracts the image's maximum
+  dB value (``src/mach/_vis.py:17-40``) -> ``envelope64``, ``db64``, ``db_zero64``.
+
 Two edge decisions shared with the HIP kernel (the reference's behaviour there is a bug or undefined):
 
 * nearest neighbour at s in (n-1, n-0.5] rounds to n when n is even (``kernel.cu:117-122`` reads one
@@ -37,7 +45,9 @@ Pinning: every function here is checked by ``tests/test_oracle.py`` against clos
 answers (Pythagorean geometries, exact interpolation of affine/quadratic signals, Tukey closed forms,
 aperture straddles, single-scatterer echoes), invariants (linearity, frame permutation, compounding
 equals the sum of single transmits, accumulation) and a scalar brute-force loop; and against the
-independent C restatement in ``oracle/das_oracle.c``.
+independent C restatement in ``oracle/das_oracle.c``. The post-processing functions are pinned by
+closed forms (Parseval for the power sum of a pure tone, Pythagorean magnitudes, exact decades in dB,
+the epsilon substitution at zero).
 """
 
 from __future__ import annotations
@@ -325,3 +335,27 @@ def bmode_db_diff(y, y_ref, floor_db=-60.0) -> float:
         db = 20 * np.log10(b / b.max())
     mask = db > floor_db
     return float(np.max(np.abs(da[mask] - db[mask]))) if mask.any() else 0.0
+
+
+def power_doppler64(y) -> np.ndarray:
+    """sum_f |y[v, f]|^2 in float64 (``examples/doppler.py:252``)."""
+    y = np.asarray(y).astype(np.complex128)
+    return (y.real ** 2 + y.imag ** 2).sum(axis=-1)
+
+
+def envelope64(y) -> np.ndarray:
+    """|y| in float64: complex magnitude for IQ, absolute value for RF (``_vis.py:24-28``)."""
+    return np.abs(np.asarray(y).astype(np.complex128))
+
+
+def db64(x, power_mode=False) -> np.ndarray:
+    """``_vis.db`` (``src/mach/_vis.py:17-28``): exact zeros -> float64 eps, then 10 or 20 log10 |x|."""
+    x = np.asarray(x, dtype=np.float64)
+    x = np.where(x == 0, np.finfo(float).eps, x)
+    return (10.0 if power_mode else 20.0) * np.log10(np.abs(x))
+
+
+def db_zero64(x, power_mode=False) -> np.ndarray:
+    """``_vis.db_zero`` (``src/mach/_vis.py:30-40``): ``db64`` minus its maximum."""
+    d = db64(x, power_mode)
+    return d - d.max()

@@ -11,10 +11,14 @@ def test_candidates_divide_frames_and_encode():
         assert cands, frames
         for lay in cands:
             lf, k = autotune.decode(lay)
+            ch = autotune.chunks_per_lane(lay)
             assert (lf in autotune.LANES or lf == 0) and k in autotune.FRAMES_PER_LANE
             assert frames % k == 0 and (lf != 0 or frames % (64 * k) == 0)
-            assert autotune.encode(lf, k) == lay
+            assert ch == 1 or ((lf, k) in autotune.MULTI_GROUP and lf * k * ch <= frames)
+            assert autotune.encode(lf, k, ch) == lay
     assert 0x404 in autotune.candidates(256)  # 64 lanes x 4 frames
+    assert 0x2104 in autotune.candidates(256)  # 16 lanes x 4 frames x 4 groups = 256 frames
+    assert 0x1104 in autotune.candidates(128) and 0x2104 not in autotune.candidates(128)
     assert 0x2 in autotune.candidates(256) and 0x2 not in autotune.candidates(8)  # band kernel
 
 

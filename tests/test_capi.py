@@ -28,7 +28,8 @@ def declared_functions():
 def test_header_declares_the_entry_points():
     names = declared_functions()
     for required in ("mach_beamform", "mach_beamform_c64", "mach_beamform_f32", "mach_last_error",
-                     "mach_abi_version", "mach_hip_runtime_version", "mach_device_arch"):
+                     "mach_abi_version", "mach_hip_runtime_version", "mach_device_arch", "mach_to_db",
+                     "mach_batch_create", "mach_batch_submit", "mach_batch_destroy"):
         assert required in names
 
 
@@ -44,7 +45,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_and_version():
     lib = _lib.load()
-    assert lib.mach_abi_version() == 1
+    assert lib.mach_abi_version() == 2
     v = lib.mach_hip_runtime_version().decode()
     major, minor, _ = v.split(".")
     assert int(major) >= 6 and int(minor) >= 0
@@ -122,3 +123,23 @@ def test_layout_hint_is_validated():
     a.channel_data = a.rx_coords_m = a.scan_coords_m = a.tx_wave_arrivals_s = 256
     assert lib.mach_beamform(ctypes.byref(a), None) == 1
     assert "layout" in _lib.last_error()
+
+
+def test_output_kind_and_peak_are_validated():
+    lib = _lib.load()
+    assert lib.mach_beamform(ctypes.byref(_args(output=7)), None) == 1
+    assert "unsupported output kind" in _lib.last_error()
+    assert lib.mach_beamform(ctypes.byref(_args(output=2, accumulate=1)), None) == 1  # envelope
+    assert "envelope" in _lib.last_error()
+    assert lib.mach_beamform(ctypes.byref(_args(output=0, peak=256)), None) == 1
+    assert "peak" in _lib.last_error()
+    # the power output needs `out` even without frames (a sum over nothing is 0)
+    assert lib.mach_beamform(ctypes.byref(_args(output=1, accumulate=0, n_frames=0)), None) == 1
+    assert "out is NULL" in _lib.last_error()
+
+
+def test_to_db_validates_pointers():
+    lib = _lib.load()
+    assert lib.mach_to_db(None, 0, None, 0, None) == 0  # nothing to convert
+    assert lib.mach_to_db(None, 4, None, 1, None) == 1
+    assert "NULL" in _lib.last_error()

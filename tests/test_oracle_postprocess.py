@@ -1,0 +1,56 @@
+"""Pins the oracle's post-processing restatements (power Doppler, envelope, dB) with closed forms.
+
+References: power Doppler ``np.square(np.abs(y)).sum(axis=-1)`` (examples/doppler.py:252);
+``db`` / ``db_zero`` (src/mach/_vis.py:17-40).
+"""
+
+import numpy as np
+import pytest
+
+from oracle.das64 import db64, db_zero64, envelope64, power_doppler64
+
+
+def test_power_of_a_pure_tone_is_amplitude_squared_times_frames():
+    # Parseval: |a e^{j w f}|^2 = |a|^2 for every frame, whatever the Doppler frequency
+    n = 256
+    f = np.arange(n)
+    a = np.array([3 + 4j, 0.5j, -2.0])
+    y = a[:, None] * np.exp(1j * 2 * np.pi * np.array([0.1, 0.37, 0.0])[:, None] * f[None, :])
+    np.testing.assert_allclose(power_doppler64(y), np.abs(a) ** 2 * n, rtol=1e-12)
+
+
+def test_power_of_rf_is_sum_of_squares_and_ignores_sign():
+    y = np.array([[1.0, -2.0, 3.0], [0.0, 0.0, 0.0]], dtype=np.float32)
+    np.testing.assert_array_equal(power_doppler64(y), [14.0, 0.0])
+
+
+def test_power_drops_a_wrong_axis_or_missing_square():
+    # a plausible slip (summing over voxels, or |y| instead of |y|^2) fails this case
+    y = np.array([[3 + 4j, 0j], [1j, 1 + 0j]])
+    np.testing.assert_array_equal(power_doppler64(y), [25.0, 2.0])
+
+
+def test_envelope_is_complex_magnitude_and_rf_absolute_value():
+    np.testing.assert_array_equal(envelope64(np.array([3 + 4j, -5 - 12j, 0j])), [5.0, 13.0, 0.0])
+    np.testing.assert_array_equal(envelope64(np.array([-2.5, 0.0, 7.0], dtype=np.float32)), [2.5, 0.0, 7.0])
+
+
+def test_db_decades_and_power_mode():
+    np.testing.assert_allclose(db64([1.0, 10.0, 0.1, 1000.0]), [0.0, 20.0, -20.0, 60.0], atol=1e-12)
+    np.testing.assert_allclose(db64([1.0, 10.0, 0.01], power_mode=True), [0.0, 10.0, -20.0], atol=1e-12)
+
+
+def test_db_replaces_exact_zero_by_float64_epsilon():
+    eps = np.finfo(float).eps
+    assert db64([0.0])[0] == pytest.approx(20 * np.log10(eps), rel=1e-15)
+    assert db64([0.0], power_mode=True)[0] == pytest.approx(10 * np.log10(eps), rel=1e-15)
+    # a tiny non-zero value is kept, not floored
+    assert db64([1e-20])[0] == pytest.approx(-400.0, rel=1e-12)
+
+
+def test_db_zero_normalises_to_the_maximum():
+    d = db_zero64(np.array([[2.0, 20.0], [0.2, 0.0]]))
+    np.testing.assert_allclose(d[:, :3].ravel()[:3], [-20.0, 0.0, -40.0], atol=1e-12)
+    assert d.max() == 0.0
+    # all-zero image: every value is epsilon, so the image is 0 dB everywhere
+    np.testing.assert_array_equal(db_zero64(np.zeros(4)), np.zeros(4))
