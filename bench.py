@@ -171,9 +171,10 @@ def copy_calibration(mib: int, reps: int = 5):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="doppler", choices=["doppler", "plane_wave", "compound", "volumetric"])
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="doppler",
+                    choices=["doppler", "plane_wave", "compound", "volumetric", "pymust_disk"])
     ap.add_argument("--storage", default=None, choices=["c64", "fp16"])
     ap.add_argument("--output", default="signal", choices=["signal", "power", "envelope"],
                     help="what the kernel writes: the per-frame signal (the reference's output, default), the "
@@ -234,6 +235,8 @@ def main(argv=None):
                 "plane_wave": "single 0deg plane wave: 128 ch x 2048 samples x 1 frame, 256x256 voxels, linear, f#1.7",
                 "compound": "75-angle plane-wave compounding: 75 tx x 128 ch x 2048 samples, 387x609 voxels, linear",
                 "volumetric": "32x32 matrix, 5 diverging tx x 1024 ch x 1536 samples x 8 frames, 128^3 voxels, linear, f#1.5",
+                "pymust_disk": "shape of the reference's published benchmark: 128 ch x 1024 samples x 32 frames "
+                               "complex64 IQ, 251x251 voxels, linear, f#1.0, no taper (synthetic rotating disk)",
             }[args.workload],
             "storage": storage,
             "output": args.output,

@@ -48,6 +48,27 @@ typedef enum mach_output_kind {
     MACH_OUTPUT_ENVELOPE = 2  /* out float32 [n_vox][n_frames]: |y| (B-mode envelope)              */
 } mach_output_kind;
 
+/* Transmit wavefront kinds for on-device arrival generation (mach_wave). */
+typedef enum mach_wave_kind {
+    MACH_WAVE_PLANE = 0,     /* distance = (p - origin) . direction (src/mach/wavefront.py:8-50)       */
+    MACH_WAVE_SPHERICAL = 1  /* distance = |origin - focus| sign(focus_z - origin_z)
+                                         - |focus - p| sign(focus_z - p_z) (src/mach/wavefront.py:53-116) */
+} mach_wave_kind;
+
+/* One transmit's wavefront. The arrival of voxel p is float32(distance(p) / sound_speed_m_s +
+ * time_offset_s), evaluated in float64 from the float32 voxel coordinates -- the value the reference's
+ * callers compute on the host (tests/compare/test_pymust.py:69-95; src/mach/io/uff.py:37-67 and
+ * 156-158, where time_offset_s is the per-transmit sequence delay) and pass as tx_wave_arrivals_s. */
+typedef struct mach_wave {
+    int32_t kind;            /* mach_wave_kind                                                  */
+    int32_t reserved;        /* 0                                                               */
+    double origin_m[3];      /* plane: wave origin; spherical: transmitting element / reference */
+    double vector_m[3];      /* plane: unit propagation direction; spherical: focus (virtual source
+                                behind the origin for a diverging wave)                          */
+    double sound_speed_m_s;
+    double time_offset_s;
+} mach_wave;
+
 typedef enum mach_status {
     MACH_OK = 0,
     MACH_ERR_INVALID_ARGUMENT = 1, /* reference: std::runtime_error (kernel.cu:740-769)       */
@@ -105,6 +126,9 @@ typedef struct mach_beamform_args {
                                          maximum value written to out (set to 0 first, then
                                          raised with an order-independent atomic max). Calls
                                          that share one `peak` must not overlap.                 */
+    const mach_wave *tx_waves;        /* optional device [n_tx] wavefronts: the kernel computes each
+                                         voxel's transmit arrival instead of reading
+                                         tx_wave_arrivals_s, which must then be NULL.            */
 } mach_beamform_args;
 
 /* Complex64 (IQ) entry. Replaces the nanobind overload `_cuda_impl.beamform` for complex64
@@ -128,6 +152,13 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
 /* General entry (transmit compounding, FP16 IQ storage, overwrite or accumulate). Replaces
  * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209. */
 int mach_beamform(const mach_beamform_args *args, void *stream);
+
+/* Materialises the transmit arrivals of `n_tx` device wavefronts for `n_vox` device voxel
+ * coordinates into device float32 out[n_tx][n_vox], with the same values the kernel computes from
+ * mach_beamform_args.tx_waves. Replaces the host-side wavefront.plane / wavefront.spherical / c
+ * step of the reference's callers (src/mach/wavefront.py:8-116, src/mach/io/uff.py:37-67). */
+int mach_tx_arrivals(const mach_wave *tx_waves, uint64_t n_tx, const float *scan_coords_m, uint64_t n_vox,
+                     float *out, void *stream);
 
 /* In-place logarithmic compression of a POWER or ENVELOPE output: x[i] = k log10(a(x[i])) -
  * k log10(a(*peak)) with a(x) = |x|, or float64 machine epsilon where x == 0, and k = 10 for power

@@ -4,14 +4,15 @@ Drop-in for the reference package ``mach`` (``src/mach/__init__.py:10-31``): ``b
 ``kernel``, ``experimental``, ``wavefront`` and ``geometry`` keep the reference's names and
 behaviour; the compute runs in hand-written HIP kernels behind the C ABI of ``include/mach_hip.h``.
 ``import mach_amd as mach`` is the intended use. Beyond the reference: ``postprocess`` (power
-Doppler and B-mode images computed in the kernel's epilogue), ``batch`` (many requests per persistent
-launch), ``serving`` (continuous batching), ``autotune`` and ``phantoms`` (synthetic workloads).
+Doppler and B-mode images computed in the kernel's epilogue), ``transmit`` (transmit arrivals evaluated
+on the device from wavefront descriptors), ``batch`` (many requests per persistent launch), ``serving``
+(continuous batching), ``autotune`` and ``phantoms`` (synthetic workloads).
 """
 
 __version__ = "0.1.3"
 
 from . import geometry, probes, wavefront  # noqa: E402  (host-only helpers, no native code)
-from . import experimental, kernel, postprocess  # noqa: E402  (load the native library; fail loudly if missing)
+from . import experimental, kernel, postprocess, transmit  # noqa: E402  (load the native library; fail loudly)
 from .kernel import InterpolationType, beamform  # noqa: E402
 
 __all__ = [
@@ -23,5 +24,6 @@ __all__ = [
     "kernel",
     "postprocess",
     "probes",
+    "transmit",
     "wavefront",
 ]

@@ -183,18 +183,20 @@ OUTPUT_SIGNAL, OUTPUT_POWER, OUTPUT_ENVELOPE = 0, 1, 2
 
 def make_args(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
               sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate=True, layout=0,
-              order=None, output=OUTPUT_SIGNAL, peak=None) -> "_lib.BeamformArgs":
+              order=None, output=OUTPUT_SIGNAL, peak=None, tx_waves=None) -> "_lib.BeamformArgs":
     """``mach_beamform_args`` for device tensors (shapes already checked). The frame count is the
-    channel data's."""
+    channel data's. ``arrivals`` is None when ``tx_waves`` (packed device wavefronts) is given."""
     core = _core_shape(cd, kind)
     return _lib.BeamformArgs(
-        cd.data_ptr(), rx.data_ptr(), scan.data_ptr(), arrivals.data_ptr(), out.data_ptr(),
+        cd.data_ptr(), rx.data_ptr(), scan.data_ptr(), arrivals.data_ptr() if arrivals is not None else None,
+        out.data_ptr(),
         n_tx, rx.shape[0], core[-2], core[-1], scan.shape[0],
         kind, int(interp_type),
         f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s, modulation_freq_hz, tukey_alpha,
         1 if accumulate else 0, int(layout),
         order.data_ptr() if order is not None else None,
         int(output), peak.data_ptr() if peak is not None else None,
+        tx_waves.data_ptr() if tx_waves is not None else None,
     )
 
 

@@ -56,6 +56,7 @@ class BeamformArgs(ctypes.Structure):
         ("voxel_order", ctypes.c_void_p),
         ("output", ctypes.c_int32),
         ("peak", ctypes.c_void_p),
+        ("tx_waves", ctypes.c_void_p),
     ]
 
 
@@ -77,6 +78,8 @@ SIGNATURES = {
     "mach_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "mach_to_db": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int32,
                                   ctypes.c_void_p]),
+    "mach_tx_arrivals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
     "mach_last_error": (ctypes.c_char_p, []),
     "mach_abi_version": (ctypes.c_int32, []),
     "mach_hip_runtime_version": (ctypes.c_char_p, []),

@@ -204,10 +204,10 @@ def _main():
     from . import phantoms
 
     ap = argparse.ArgumentParser(description="Tune kernel layouts for the benchmark workloads and save the table")
-    ap.add_argument("--workloads", default="plane_wave,compound,doppler,volumetric")
+    ap.add_argument("--workloads", default="plane_wave,compound,doppler,volumetric,pymust_disk")
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--power", default="doppler", help="workloads also tuned for the power Doppler output")
+    ap.add_argument("--power", default="doppler,pymust_disk", help="workloads also tuned for the power Doppler output")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     for name in args.workloads.split(","):

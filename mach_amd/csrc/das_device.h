@@ -77,6 +77,26 @@ __device__ __forceinline__ PairGeom pair_geom(const DasParams &p, double xv, dou
     return g;
 }
 
+// Transmit arrival of voxel (x, y, z) (float64 values of its float32 coordinates) under wavefront `w`:
+// float32(distance / c + offset), the value the reference's callers compute on the host and pass as
+// tx_wave_arrivals_s (src/mach/wavefront.py:8-116 divided by the sound speed; src/mach/io/uff.py:156-158
+// adds the per-transmit sequence delay). Returned as float64 of that float32, like a read of the array.
+__device__ __forceinline__ double wave_arrival(const WaveDesc &w, double x, double y, double z) {
+    double d;
+    if (w.kind == kWavePlane) {
+        d = ((x - w.origin[0]) * w.vec[0] + (y - w.origin[1]) * w.vec[1]) + (z - w.origin[2]) * w.vec[2];
+    } else {
+        const double ax = w.origin[0] - w.vec[0], ay = w.origin[1] - w.vec[1], az = w.origin[2] - w.vec[2];
+        const double bx = w.vec[0] - x, by = w.vec[1] - y, bz = w.vec[2] - z;
+        const double origin_focus = sqrt((ax * ax + ay * ay) + az * az);
+        const double focus_point = sqrt((bx * bx + by * by) + bz * bz);
+        const double so = (w.vec[2] > w.origin[2]) ? 1.0 : ((w.vec[2] < w.origin[2]) ? -1.0 : 0.0);
+        const double sp = (w.vec[2] > z) ? 1.0 : ((w.vec[2] < z) ? -1.0 : 0.0);
+        d = origin_focus * so - focus_point * sp;
+    }
+    return (double)(float)(d / w.c + w.offset);
+}
+
 // Transmit-dependent part: tau = T[v] + d / c, skips (kernel.cu:634), sample index, interpolation
 // taps (kernel.cu:106-252) and the complex coefficient w * exp(j 2 pi f_mod tau) (kernel.cu:648-661).
 template <int INTERP>
@@ -329,8 +349,12 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             const int nt = (int)((p.n_tx - t0) < TC ? (p.n_tx - t0) : TC);
             double Tv[TC];
 #pragma unroll
-            for (int tt = 0; tt < TC; ++tt)
-                Tv[tt] = (vok && tt < nt) ? (double)p.arrivals[(t0 + tt) * p.n_vox + v] : 0.0;
+            for (int tt = 0; tt < TC; ++tt) {
+                Tv[tt] = 0.0;
+                if (vok && tt < nt)
+                    Tv[tt] = p.waves != nullptr ? wave_arrival(p.waves[t0 + tt], xv, yv, zv)
+                                                : (double)p.arrivals[(t0 + tt) * p.n_vox + v];
+            }
             for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
                 const int64_t e = e0 + lf;
                 PairSetup s[TC];

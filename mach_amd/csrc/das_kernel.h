@@ -15,6 +15,17 @@ constexpr int kTxChunk = 8;
 
 // Everything a launch needs, already validated and converted. Geometry scalars are float64 values
 // of the float32 API scalars.
+// One transmit's wavefront (layout of mach_wave in include/mach_hip.h).
+enum WaveKind { kWavePlane = 0, kWaveSpherical = 1 };
+struct WaveDesc {
+    int32_t kind;
+    int32_t reserved;
+    double origin[3];
+    double vec[3];  // plane: unit direction; spherical: focus
+    double c;
+    double offset;
+};
+
 // What the kernel writes (mach_output_kind): the beamformed signal [n_vox][n_frames], the power
 // Doppler image sum_f |y|^2 [n_vox] (float32), or the envelope |y| [n_vox][n_frames] (float32).
 enum OutputKind { kOutSignal = 0, kOutPower = 1, kOutEnvelope = 2 };
@@ -23,7 +34,8 @@ struct DasParams {
     const void *data;        // [n_tx][n_rx][n_samples][n_frames] of the storage type
     const float *rx;         // [n_rx][3]
     const float *vox;        // [n_vox][3]
-    const float *arrivals;   // [n_tx][n_vox]
+    const float *arrivals;   // [n_tx][n_vox], or nullptr when `waves` is given
+    const WaveDesc *waves;   // optional [n_tx] wavefronts: arrivals computed per voxel in the kernel
     void *out;               // [n_vox][n_frames]
     const int32_t *order;    // optional [n_vox] processing order (nullptr = identity)
     int64_t n_tx, n_rx, n_samples, n_frames, n_vox;
@@ -64,15 +76,23 @@ inline DasLayout decode_layout(int32_t code) {
     return DasLayout{(code >> 4) & 0xff, code & 0xf, ch_log2 <= 2 ? 1 << ch_log2 : 0};
 }
 
-// Block-to-XCD remap of a plain launch with `blocks_y` frame-block rows (xcd_remap). Measured on gfx950
-// with depth-major voxel order: a launch with one block row and few transmits is limited by the
-// deepest XCD (work grows with depth), so interleaving runs of 8 blocks balances it -- volumetric
-// 50.5 -> 39.1 ms, Doppler power image 0.73 -> 0.49 ms, single plane wave 0.057 -> 0.054 ms --
-// while launches with several block rows (Doppler signal output: 0.558 vs 0.576 ms) or many
-// transmits (75-angle compounding: 5.40 vs 5.61 ms) keep more L2 reuse with contiguous eighths.
+// Block-to-XCD remap of a plain launch with `blocks_y` frame-block rows (xcd_remap). Work grows with
+// depth (the f-number aperture widens), so with a depth-major voxel order and contiguous eighths the
+// deepest XCD finishes last unless several block rows mix the depths. Measured on gfx950 (ms):
+//   launch                                   contiguous  interleaved runs of 8
+//   Doppler c64 signal, 32x2x2 (2 rows)        0.637        0.432
+//   Doppler c64 signal, 16x4x2 (2 rows)        0.661        0.439
+//   Doppler c64 power image, 32x4 (1 row)      0.644        0.401
+//   volume, 2x4, 5 transmits (1 row)          50.5         39.1
+//   single plane wave, 4x1 (1 row)             0.057        0.054
+//   Doppler c64 signal, 32x2 (4 rows)          0.493        0.515
+//   Doppler fp16 signal, 16x4 (4 rows)         0.340        0.362
+//   75-angle compounding, 1x1 (1 row)          5.40         5.61
+// so runs are interleaved for launches of at most two rows and fewer transmits than a chunk; with
+// many transmits the L2 reuse of contiguous eighths wins.
 inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_y) {
     if (p.xcd_group >= 0) return (uint32_t)p.xcd_group;
-    return (blocks_y == 1 && p.n_tx < kTxChunk) ? 8u : 0u;
+    return (blocks_y <= 2 && p.n_tx < kTxChunk) ? 8u : 0u;
 }
 
 // Picks the layout for a problem (frame count and pointer alignment decide vector widths).

@@ -6,6 +6,7 @@
 #include <stdio.h>
 
 #include <climits>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -21,6 +22,10 @@
 struct mach_batch {
     mach::BatchState *state;
 };
+
+static_assert(sizeof(mach_wave) == sizeof(mach::WaveDesc), "mach_wave and WaveDesc must have one layout");
+static_assert(offsetof(mach_wave, vector_m) == offsetof(mach::WaveDesc, vec), "mach_wave layout");
+static_assert(offsetof(mach_wave, time_offset_s) == offsetof(mach::WaveDesc, offset), "mach_wave layout");
 
 namespace {
 
@@ -80,8 +85,10 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     if (need_data && a->out == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: out is NULL");
     if (need_data && a->scan_coords_m == nullptr)
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: scan_coords_m is NULL");
+    if (a->tx_waves != nullptr && a->tx_wave_arrivals_s != nullptr)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: give tx_wave_arrivals_s or tx_waves, not both");
     const bool need_inputs = need_data && a->n_tx > 0 && a->n_rx > 0;
-    if (need_inputs && (a->rx_coords_m == nullptr || a->tx_wave_arrivals_s == nullptr))
+    if (need_inputs && (a->rx_coords_m == nullptr || (a->tx_wave_arrivals_s == nullptr && a->tx_waves == nullptr)))
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: rx_coords_m / tx_wave_arrivals_s is NULL");
     if (need_inputs && a->n_samples > 0 && a->n_frames > 0 && a->channel_data == nullptr)
         return fail(MACH_ERR_INVALID_ARGUMENT, "Error: channel_data is NULL");
@@ -90,6 +97,7 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->rx = a->rx_coords_m;
     p->vox = a->scan_coords_m;
     p->arrivals = a->tx_wave_arrivals_s;
+    p->waves = (const mach::WaveDesc *)a->tx_waves;
     p->out = a->out;
     p->order = a->voxel_order;
     p->n_tx = (int64_t)a->n_tx;
@@ -137,7 +145,8 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
         return MACH_OK;
     }
     const mach::DasLayout l = mach::decode_layout(a->layout);
-    if (l.lanes_per_voxel == 0 && p.output != mach::kOutSignal) {  // epilogues live in the general kernel
+    // epilogues and on-device arrivals live in the general kernel
+    if (l.lanes_per_voxel == 0 && (p.output != mach::kOutSignal || p.waves != nullptr)) {
         out->layout = mach::choose_layout(p, a->data_type);
         return MACH_OK;
     }
@@ -290,6 +299,20 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
     return beamform_single(MACH_DATA_F32, channel_data, rx_coords_m, scan_coords_m, tx_wave_arrivals_s, out, n_rx,
                            n_samples, n_frames, n_vox, f_number, rx_start_s, sampling_freq_hz, sound_speed_m_s,
                            modulation_freq_hz, tukey_alpha, interp_type, stream);
+}
+
+int mach_tx_arrivals(const mach_wave *tx_waves, uint64_t n_tx, const float *scan_coords_m, uint64_t n_vox,
+                     float *out, void *stream) {
+    g_last_error.clear();
+    if (n_tx == 0 || n_vox == 0) return MACH_OK;
+    if (tx_waves == nullptr || scan_coords_m == nullptr || out == nullptr)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: tx_waves / scan_coords_m / out is NULL");
+    if (n_tx > (uint64_t)INT32_MAX || n_vox > (uint64_t)INT64_MAX / 4 / n_tx)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: array dimensions exceed the supported range");
+    const hipError_t e = mach::launch_tx_arrivals((const mach::WaveDesc *)tx_waves, n_tx, scan_coords_m, n_vox, out,
+                                                  (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mach_tx_arrivals");
+    return MACH_OK;
 }
 
 int mach_to_db(float *x, uint64_t n, const float *peak, int32_t power_mode, void *stream) {

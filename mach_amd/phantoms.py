@@ -16,6 +16,10 @@ Workloads (grid/array/pulse conventions follow ``src/mach/io/must.py:71-104`` of
   compound           75 plane waves -16..+16 deg, [75 x 128 x 2048 x 1], 387x609 grid, 3 cysts
   doppler            rotating disk + static clutter, [128 x 1024 x 256], 256x256 grid, per-frame output
   volumetric         32x32 matrix, 5 diverging waves, [5 x 1024 x 1536 x 8], 128^3 grid, FP16 IQ
+  pymust_disk        the shape of the reference's own published benchmark (BENCHMARKS.md:22-33,
+                     tests/compare/conftest.py:25-35, test_pymust.py:129-178): L7-4-like 128-element
+                     array, rotating disk, [128 x 1024 x 32], 251x251 grid, linear, f-number 1, no
+                     apodization taper; synthetic data (the PyMUST recording is a download)
 Each takes a ``scale`` that shrinks grid axes, element and scatterer counts for quick tests (the
 sample record, hence the imaged depth, is kept).
 """
@@ -197,6 +201,10 @@ def workload_grid(name: str, grid_scale: float = 1.0, n_tx: int | None = None):
         nx, nz = max(8, int(round(387 * grid_scale))), max(8, int(round(609 * grid_scale)))
         scan, gshape = _grid(nx, nz, (-19e-3, 19e-3), (5e-3, 50e-3))
         return scan, gshape, plane_wave_arrivals(scan, _compound_angles(75 if n_tx is None else n_tx))
+    if name == "pymust_disk":
+        n = max(8, int(round(251 * grid_scale)))
+        scan, gshape = _grid(n, n, (-12.5e-3, 12.5e-3), (10e-3, 35e-3))
+        return scan, gshape, plane_wave_arrivals(scan, [0.0], c=1480.0)
     if name == "volumetric":
         n = max(4, int(round(128 * grid_scale)))
         x = np.linspace(-10e-3, 10e-3, n)
@@ -326,7 +334,45 @@ def volumetric(device="cpu", seed=3, scale=1.0, n_frames=8) -> Acquisition:
     return _finish("volumetric", data, rx, scan, arrivals, (n_grid, n_grid, n_grid), _base_params(1.5, 1), seed, device)
 
 
-WORKLOADS = {"plane_wave": plane_wave, "compound": compound, "doppler": doppler, "volumetric": volumetric}
+def pymust_disk(device="cpu", seed=5, scale=1.0, n_frames=32) -> Acquisition:
+    """Synthetic stand-in with the shape of the reference's published benchmark (PyMUST rotating-disk
+    recording, BENCHMARKS.md:22-33): 128 elements at 0.298 mm pitch (L7-4), f0 5.208 MHz, fs = 4 f0,
+    c 1480 m/s, one 0 deg plane wave per frame, 32 frames, 1024 samples (35 mm and back), grid 251 x 251
+    over x +-12.5 mm, z 10-35 mm (tests/compare/conftest.py:25-35), linear interpolation, f-number 1.0,
+    tukey_alpha 0 (test_pymust.py:146-175). Rotating disk (r = 8 mm, centre z = 22.5 mm) of 2000
+    scatterers in +20 dB static clutter."""
+    rng = np.random.default_rng(seed)
+    f0, c, pitch = 5.208e6, 1480.0, 0.298e-3
+    fs = 4 * f0
+    n_el = max(8, int(round(128 * scale)))
+    n_s = 1024
+    n_grid = max(8, int(round(251 * scale)))
+    n_disk = max(8, int(round(2000 * scale)))
+    n_clutter = max(8, int(round(2000 * scale)))
+    rx = probes.linear_probe_positions(n_el, pitch)
+    radius, cz, omega, prf = 8e-3, 22.5e-3, 2.0, 5000.0
+    rr = radius * np.sqrt(rng.uniform(0, 1, n_disk))
+    th = rng.uniform(0, 2 * np.pi, n_disk)
+    amps_d = (rng.standard_normal(n_disk) + 1j * rng.standard_normal(n_disk)) / math.sqrt(2)
+    fr = np.arange(n_frames)
+    ang = th[None, :] + omega * fr[:, None] / prf
+    disk = np.stack([rr * np.cos(ang), np.zeros_like(ang), cz + rr * np.sin(ang)], -1)
+    clutter = np.stack([rng.uniform(-12.5e-3, 12.5e-3, n_clutter), np.zeros(n_clutter),
+                        rng.uniform(10e-3, 35e-3, n_clutter)], -1)
+    amps_c = 10.0 * (rng.standard_normal(n_clutter) + 1j * rng.standard_normal(n_clutter)) / math.sqrt(2)
+    txd = lambda p, t: _tx_delay_plane(p, 0.0, c)  # noqa: E731
+    kw = dict(fs=fs, f0=f0, c=c, device=device)
+    data = synthesize_echoes(disk, amps_d, rx, txd, 1, n_s, **kw)
+    data = data + synthesize_echoes(clutter[None], amps_c, rx, txd, 1, n_s, **kw)
+    data = _add_noise(data, rng)
+    scan, gshape, arrivals = workload_grid("pymust_disk", n_grid / 251)
+    params = dict(f_number=1.0, rx_start_s=0.0, sampling_freq_hz=fs, sound_speed_m_s=c, modulation_freq_hz=f0,
+                  tukey_alpha=0.0, interp_type=1)
+    return _finish("pymust_disk", data, rx, scan, arrivals, gshape, params, seed, device)
+
+
+WORKLOADS = {"plane_wave": plane_wave, "compound": compound, "doppler": doppler, "volumetric": volumetric,
+             "pymust_disk": pymust_disk}
 
 
 def make(name: str, device="cpu", **kw) -> Acquisition:

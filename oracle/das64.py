@@ -35,6 +35,12 @@ Post-processing the reference leaves to its callers, restated for the kernel's e
   10 log10 (power) or 20 log10 (amplitude) of the magnitude; ``db_zero`` subtracts the image's maximum
   dB value (``src/mach/_vis.py:17-40``) -> ``envelope64``, ``db64``, ``db_zero64``.
 
+Transmit arrivals from wavefront descriptors (``wave_arrivals64``): float32(distance / c + offset),
+distance per ``src/mach/wavefront.py:8-50`` (plane: (p - o) . d) and ``53-116`` (spherical:
+|o - f| sign(f_z - o_z) - |f - p| sign(f_z - p_z)), evaluated in float64 from the float32 voxel
+coordinates with an explicit summation order ((x + y) + z), as the reference's callers do on the host
+(``src/mach/io/uff.py:37-67, 156-158``).
+
 Two edge decisions shared with the HIP kernel (the reference's behaviour there is a bug or undefined):
 
 * nearest neighbour at s in (n-1, n-0.5] rounds to n when n is even (``kernel.cu:117-122`` reads one
@@ -359,3 +365,24 @@ def db_zero64(x, power_mode=False) -> np.ndarray:
     """``_vis.db_zero`` (``src/mach/_vis.py:30-40``): ``db64`` minus its maximum."""
     d = db64(x, power_mode)
     return d - d.max()
+
+
+def wave_arrivals64(waves, scan) -> np.ndarray:
+    """[T, V] float32 transmit arrivals of wavefronts ``waves`` = [(kind, origin[3], vector[3], c, offset)]
+    (kind 0 plane with unit direction ``vector``, 1 spherical with focus ``vector``) at float32 voxels."""
+    p = np.asarray(scan, dtype=np.float32).astype(np.float64)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    out = []
+    for kind, origin, vec, c, offset in waves:
+        o = np.asarray(origin, dtype=np.float64)
+        v = np.asarray(vec, dtype=np.float64)
+        if kind == 0:
+            d = ((x - o[0]) * v[0] + (y - o[1]) * v[1]) + (z - o[2]) * v[2]
+        else:
+            a = o - v
+            origin_focus = np.sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2])
+            bx, by, bz = v[0] - x, v[1] - y, v[2] - z
+            focus_point = np.sqrt((bx * bx + by * by) + bz * bz)
+            d = origin_focus * np.sign(v[2] - o[2]) - focus_point * np.sign(v[2] - z)
+        out.append((d / float(c) + float(offset)).astype(np.float32))
+    return np.stack(out) if out else np.zeros((0, len(p)), dtype=np.float32)

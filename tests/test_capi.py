@@ -143,3 +143,14 @@ def test_to_db_validates_pointers():
     assert lib.mach_to_db(None, 0, None, 0, None) == 0  # nothing to convert
     assert lib.mach_to_db(None, 4, None, 1, None) == 1
     assert "NULL" in _lib.last_error()
+
+
+def test_arrivals_and_waves_are_exclusive():
+    lib = _lib.load()
+    a = _args(accumulate=0)
+    a.out = a.channel_data = a.rx_coords_m = a.scan_coords_m = a.tx_wave_arrivals_s = 256  # never dereferenced
+    a.tx_waves = 512
+    assert lib.mach_beamform(ctypes.byref(a), None) == 1
+    assert "not both" in _lib.last_error()
+    assert lib.mach_tx_arrivals(None, 0, None, 5, None, None) == 0  # nothing to compute
+    assert lib.mach_tx_arrivals(None, 2, None, 5, None, None) == 1

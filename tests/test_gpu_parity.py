@@ -233,6 +233,7 @@ def workload_parity(acq, device, voxel_sample=None, fp16=False, seed=0):
     ("compound", 0.3, 4000),
     ("doppler", 0.5, 600),
     ("volumetric", 0.25, 3000),
+    ("pymust_disk", 1.0, 2000),
 ])
 def test_workloads_fp32(device, name, scale, sample):
     log(f"workload {name} scale={scale}")

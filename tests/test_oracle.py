@@ -254,3 +254,29 @@ def test_rel_l2_metric():
     assert rel_l2(y, y) == 0.0
     assert rel_l2(y * 1.1, y) == pytest.approx(0.1)
     assert math.isfinite(rel_l2(np.zeros(2), np.zeros(2)))
+
+
+def test_wave_arrivals_match_wavefront_functions_and_closed_forms():
+    """wave_arrivals64 against the reference-behaviour wavefront module (itself pinned by the reference's
+    KATs) and closed forms: a 0 deg plane wave arrives at z / c; a diverging wave from a source d behind
+    the origin reaches depth z on axis at (z + d - d) / c = z / c; the offset adds after the division."""
+    from mach_amd import wavefront
+    from oracle.das64 import wave_arrivals64
+
+    rng = np.random.default_rng(11)
+    scan = (rng.normal(size=(500, 3)) * 1e-2 + [0, 0, 3e-2]).astype(np.float32)
+    d = np.array([math.sin(0.2), 0.0, math.cos(0.2)])
+    src = np.array([1e-3, -2e-3, -10e-3])
+    waves = [(0, (0.0, 0.0, 0.0), d, 1540.0, 0.0), (1, (0.0, 0.0, 0.0), src, 1480.0, 2e-6),
+             (0, (1e-3, 0.0, 0.0), (0.0, 0.0, 1.0), 1540.0, -1e-6)]
+    t = wave_arrivals64(waves, scan)
+    s64 = scan.astype(np.float64)
+    ref0 = wavefront.plane(np.zeros(3), s64, d) / 1540.0
+    ref1 = wavefront.spherical(np.zeros(3), s64, src) / 1480.0 + 2e-6
+    np.testing.assert_allclose(t[0], ref0.astype(np.float32), rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(t[1], ref1.astype(np.float32), rtol=1e-6, atol=1e-12)
+    np.testing.assert_array_equal(t[2], (s64[:, 2] / 1540.0 - 1e-6).astype(np.float32))
+    on_axis = np.array([[0.0, 0.0, 0.02]], dtype=np.float32)
+    t_axis = wave_arrivals64([(1, (0.0, 0.0, 0.0), (0.0, 0.0, -0.01), 1500.0, 0.0)], on_axis)[0, 0]
+    assert t_axis == np.float32(np.float64(np.float32(0.02)) / 1500.0)
+    assert t.dtype == np.float32 and t.shape == (3, 500)
