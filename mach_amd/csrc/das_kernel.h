@@ -76,23 +76,27 @@ inline DasLayout decode_layout(int32_t code) {
     return DasLayout{(code >> 4) & 0xff, code & 0xf, ch_log2 <= 2 ? 1 << ch_log2 : 0};
 }
 
-// Block-to-XCD remap of a plain launch with `blocks_y` frame-block rows (xcd_remap). Work grows with
+// Block-to-XCD remap of a plain launch of blocks_x x blocks_y blocks (xcd_remap). Work grows with
 // depth (the f-number aperture widens), so with a depth-major voxel order and contiguous eighths the
 // deepest XCD finishes last unless several block rows mix the depths. Measured on gfx950 (ms):
-//   launch                                   contiguous  interleaved runs of 8
-//   Doppler c64 signal, 32x2x2 (2 rows)        0.637        0.432
-//   Doppler c64 signal, 16x4x2 (2 rows)        0.661        0.439
-//   Doppler c64 power image, 32x4 (1 row)      0.644        0.401
-//   volume, 2x4, 5 transmits (1 row)          50.5         39.1
-//   single plane wave, 4x1 (1 row)             0.057        0.054
-//   Doppler c64 signal, 32x2 (4 rows)          0.493        0.515
-//   Doppler fp16 signal, 16x4 (4 rows)         0.340        0.362
-//   75-angle compounding, 1x1 (1 row)          5.40         5.61
-// so runs are interleaved for launches of at most two rows and fewer transmits than a chunk; with
-// many transmits the L2 reuse of contiguous eighths wins.
-inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_y) {
+//   launch                                   contiguous  runs of 8  runs of 32-128
+//   Doppler c64 signal, 32x2x2 (2 rows)        0.639       0.440      0.425 (64)
+//   Doppler c64 power image, 32x4 (1 row)      0.640       0.401      0.389 (64)
+//   Doppler fp16 power image, 64x4 (1 row)     0.416       0.271      0.257 (128)
+//   volume, 2x4, 5 transmits (1 row)          50.5        39.5       37.5 (128)
+//   single plane wave, 4x1 (1 row)             0.057       0.054
+//   Doppler c64 signal, 32x2 (4 rows)          0.493       0.515
+//   75-angle compounding, 1x1 (1 row)          5.40        5.61
+// Launches of at most two rows with fewer transmits than a chunk take runs of blocks_x / 128 blocks
+// (a power of two in [8, 128]): each XCD then sweeps the depth range in ~16 bands, balanced, while a
+// run keeps neighbouring voxels, which share channel-data rows, in one L2. With more rows or many
+// transmits the L2 reuse of contiguous eighths wins.
+inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_x, int64_t blocks_y) {
     if (p.xcd_group >= 0) return (uint32_t)p.xcd_group;
-    return (blocks_y <= 2 && p.n_tx < kTxChunk) ? 8u : 0u;
+    if (blocks_y > 2 || p.n_tx >= kTxChunk) return 0u;
+    uint32_t g = 8u;
+    while (g < 128u && (int64_t)(2 * g) * 128 <= blocks_x) g *= 2u;
+    return g;
 }
 
 // Picks the layout for a problem (frame count and pointer alignment decide vector widths).

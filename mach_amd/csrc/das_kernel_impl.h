@@ -31,7 +31,7 @@ hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks_x, (unsigned)blocks_y, 1);
     DasParams q = p;
-    q.xcd_group = (int32_t)xcd_group_for(p, blocks_y);
+    q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
     if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
         hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH>), grid, dim3(256, 1, 1), 0, stream, q);
     } else {

@@ -9,11 +9,15 @@ server:
   lands in its ring slot when it arrives, the way an acquisition front end DMAs into device memory;
 * admits arrived requests in arrival order while their channel-data and output bytes fit the HBM
   budget (the rings' capacity). Later arrivals wait until completed work frees space;
-* packs every admitted, not-yet-running request into one persistent-kernel launch (``BatchRunner``,
-  graph replay). Requests of a batch that share a geometry (probe, grid, transmits, dtype) are fused
-  into one ensemble along the frame axis, so their float64 pair geometry is computed once. Up to two
-  batches are in flight, on two streams, so requests that arrive while a batch runs are admitted and
-  start while the previous batch drains;
+* packs every admitted, not-yet-running request into one batch. Requests of a batch that share a
+  geometry (probe, grid, transmits, dtype) are fused into one ensemble along the frame axis, so their
+  float64 pair geometry is computed once. Small requests (and fused groups) go into one
+  persistent-kernel submission (``BatchRunner``, graph replay); a request or group big enough to fill
+  the GPU by itself (``plain_min_voxel_frames``) is launched as its own plain kernel on the batch's
+  stream, where it gets the launch-shape choices (frame groups, balanced XCD remap) a work queue
+  cannot make. Either way nothing synchronises per request. Up to two batches are in flight, on two
+  streams, so requests that arrive while a batch runs are admitted and start while the previous
+  batch drains;
 * reports per-request latency (arrival -> result ready on the device), p50/p99, and throughput in
   points/s over the makespan.
 
@@ -239,6 +243,7 @@ class ServeReport:
     max_batch: int = 1
     n_batches: int = 0
     n_fused: int = 0  # requests that ran fused with others of the same geometry
+    n_direct: int = 0  # requests (or fused groups' members) launched as their own plain kernel in a batch
 
     @property
     def gpts_per_s(self) -> float:
@@ -278,10 +283,11 @@ def _fusion_groups(reqs, fuse=True, max_frames=256):
 
 
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
-                  fuse=True, max_fused_frames=256):
+                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 20):
     """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
-    fused into one ensemble (``fuse``). Returns a ServeReport; with ``keep_outputs`` each request's
-    ``out`` is a private copy of its result."""
+    fused into one ensemble (``fuse``); a request or fused group of at least ``plain_min_voxel_frames``
+    voxel-frames runs as its own plain launch within the batch (``None``: never). Returns a
+    ServeReport; with ``keep_outputs`` each request's ``out`` is a private copy of its result."""
     free, _total = torch.cuda.mem_get_info(device)
     budget = min(int(budget_bytes), int(free * 0.8))
     data_ring = DeviceRing(budget * 3 // 4, device)
@@ -293,7 +299,7 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     waiting = collections.deque()
     admitted = []  # (req, copy_event)
     inflight = collections.deque()  # (reqs, event)
-    n_done, n_batches, biggest, n_fused = 0, 0, 0, 0
+    n_done, n_batches, biggest, n_fused, n_direct = 0, 0, 0, 0, 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     while n_done < len(reqs):
@@ -336,27 +342,35 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             stream = compute[n_batches % 2]
             for _r, ev in take:
                 stream.wait_event(ev)
-            breqs, keep = [], []
+            breqs, direct, keep = [], [], []
             for group in _fusion_groups([r for r, _ in take], fuse, max_fused_frames):
                 if len(group) == 1:
                     r = group[0]
                     r.out = _out_view(r, out_ring, r.out_slot)
-                    breqs.append(_to_request(r, _data_view(r, data_ring, r.data_slot), r.out))
-                    continue
-                # same geometry: one ensemble request over the concatenated frames
-                with torch.cuda.stream(stream):
-                    fused = torch.cat([_data_view(r, data_ring, r.data_slot) for r in group], dim=3)
-                    out_f = torch.empty((group[0].geometry.scan_coords_m.shape[0], fused.shape[3]),
-                                        dtype=torch.complex64, device=device)
-                a = 0
-                for r in group:
-                    nf = r.source.shape[3]
-                    r.out = out_f[:, a:a + nf]
-                    a += nf
-                breqs.append(_to_request(group[0], fused, out_f))
-                keep.append(fused)
-                n_fused += len(group)
-            runner.submit(breqs, stream=stream)
+                    req = _to_request(r, _data_view(r, data_ring, r.data_slot), r.out)
+                else:
+                    # same geometry: one ensemble request over the concatenated frames
+                    with torch.cuda.stream(stream):
+                        fused = torch.cat([_data_view(r, data_ring, r.data_slot) for r in group], dim=3)
+                        out_f = torch.empty((group[0].geometry.scan_coords_m.shape[0], fused.shape[3]),
+                                            dtype=torch.complex64, device=device)
+                    a = 0
+                    for r in group:
+                        nf = r.source.shape[3]
+                        r.out = out_f[:, a:a + nf]
+                        a += nf
+                    req = _to_request(group[0], fused, out_f)
+                    keep.append(fused)
+                    n_fused += len(group)
+                if plain_min_voxel_frames is not None and req.out.numel() >= plain_min_voxel_frames:
+                    direct.append(req)
+                    n_direct += len(group)
+                else:
+                    breqs.append(req)
+            for req in direct:  # GPU-filling requests: their own launch, tuned layout, balanced remap
+                _batch.run_plain(req, stream=stream)
+            if breqs:
+                runner.submit(breqs, stream=stream)
             ev = torch.cuda.Event()
             ev.record(stream)
             inflight.append(([r for r, _ in take], ev, keep))
@@ -371,6 +385,7 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     runner.close()
     rep = _report("batched", reqs, reqs[0].arrival_s if reqs else 0.0, biggest, n_batches)
     rep.n_fused = n_fused
+    rep.n_direct = n_direct
     del data_ring, out_ring
     return rep
 

@@ -254,7 +254,8 @@ def test_doppler_full_size_voxel_sample(device):
     assert err <= TOL_FP32
 
 
-@pytest.mark.parametrize("name,scale,sample", [("volumetric", 0.25, 3000), ("doppler", 0.5, 600)])
+@pytest.mark.parametrize("name,scale,sample", [("volumetric", 0.25, 3000), ("doppler", 0.5, 600),
+                                               ("compound", 0.3, 4000)])
 def test_workloads_fp16_storage(device, name, scale, sample):
     acq = phantoms.make(name, device=device, scale=scale)
     y16, ref16, _ = workload_parity(acq, device, sample, fp16=True)

@@ -30,20 +30,25 @@ def _reset(reqs):
         r.out = None
 
 
-@pytest.mark.parametrize("fuse", [True, False])
-def test_mix_equals_alone(device, burst_reqs, fuse):
+@pytest.mark.parametrize("fuse,direct", [(True, None), (False, None), (True, 1 << 14)])
+def test_mix_equals_alone(device, burst_reqs, fuse, direct):
+    """Each result is bitwise the same alone and in the mix: with and without fusion, and with large
+    requests launched as their own plain kernels inside a batch (``direct`` voxel-frames threshold)."""
     reqs = burst_reqs
     _reset(reqs)
     plain = serving.serve_plain(reqs, device, keep_outputs=True)
     alone = {r.rid: r.out for r in reqs}
     _reset(reqs)
-    mixed = serving.serve_batched(reqs, device, budget_bytes=8 << 30, keep_outputs=True, fuse=fuse)
+    mixed = serving.serve_batched(reqs, device, budget_bytes=8 << 30, keep_outputs=True, fuse=fuse,
+                                  plain_min_voxel_frames=direct)
     for r in reqs:
         assert torch.equal(r.out, alone[r.rid]), r.rid
     print(plain.as_dict(), mixed.as_dict(), flush=True)
     assert mixed.n_requests == plain.n_requests == len(reqs)
     assert mixed.n_batches < len(reqs)  # requests were actually batched
     assert (mixed.n_fused > 0) == fuse
+    assert (mixed.n_direct > 0) == (direct is not None)
+    assert mixed.n_direct < len(reqs)  # the small ones still went through the persistent kernel
     for rep in (plain, mixed):
         assert math.isfinite(rep.p50_ms) and rep.p99_ms >= rep.p50_ms > 0
 
