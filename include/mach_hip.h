@@ -160,6 +160,17 @@ int mach_beamform(const mach_beamform_args *args, void *stream);
 int mach_tx_arrivals(const mach_wave *tx_waves, uint64_t n_tx, const float *scan_coords_m, uint64_t n_vox,
                      float *out, void *stream);
 
+/* Analytic signal of real RF channel data, the step before beamforming IQ data: the device
+ * counterpart of the reference's UFF preprocessing
+ * (src/mach/io/uff.py:70-103), which takes scipy.signal.hilbert along the sample axis and rearranges
+ * (n_samples, n_elements, n_waves, n_frames) to (n_waves, n_elements, n_samples, n_frames).
+ * rf: device float32 [n_samples][n_rx][n_tx][n_frames]; out: device [n_tx][n_rx][n_samples][n_frames]
+ * of out_type MACH_DATA_C64 (complex64) or MACH_DATA_C32H (binary16 re, im pairs; values must fit
+ * binary16). n_samples must be 2^a 3^b 5^c and at most 4096 (MACH_ERR_UNSUPPORTED otherwise).
+ * Beamform the result with modulation_freq_hz = 0 (the analytic signal keeps its carrier). */
+int mach_analytic_signal(const float *rf, uint64_t n_samples, uint64_t n_rx, uint64_t n_tx, uint64_t n_frames,
+                         void *out, int32_t out_type, void *stream);
+
 /* In-place logarithmic compression of a POWER or ENVELOPE output: x[i] = k log10(a(x[i])) -
  * k log10(a(*peak)) with a(x) = |x|, or float64 machine epsilon where x == 0, and k = 10 for power
  * (power_mode != 0) or 20 for amplitude. Restates `db_zero` of src/mach/_vis.py:17-40 (which

@@ -78,6 +78,8 @@ SIGNATURES = {
     "mach_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "mach_to_db": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int32,
                                   ctypes.c_void_p]),
+    "mach_analytic_signal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "mach_tx_arrivals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                                         ctypes.c_void_p, ctypes.c_void_p]),
     "mach_last_error": (ctypes.c_char_p, []),

@@ -23,7 +23,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 # largest first, so the parallel build is not left waiting on one long compile at the end
 SOURCES = ["das_batch_tcn.hip", "das_batch_tc1.hip", "das_kernel_c64.hip", "das_kernel_c32h.hip",
-           "das_kernel_f32.hip", "das_band.hip", "das_batch.hip", "das_kernel.hip", "das_post.hip", "mach_capi.hip"]
+           "das_kernel_f32.hip", "das_band.hip", "das_batch.hip", "das_kernel.hip", "das_post.hip", "das_hilbert.hip",
+           "mach_capi.hip"]
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 ARCH = "gfx950"

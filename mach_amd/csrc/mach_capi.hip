@@ -16,6 +16,7 @@
 #include "../../include/mach_hip.h"
 #include "das_band.h"
 #include "das_batch.h"
+#include "das_hilbert.h"
 #include "das_kernel.h"
 #include "das_post.h"
 
@@ -312,6 +313,28 @@ int mach_tx_arrivals(const mach_wave *tx_waves, uint64_t n_tx, const float *scan
     const hipError_t e = mach::launch_tx_arrivals((const mach::WaveDesc *)tx_waves, n_tx, scan_coords_m, n_vox, out,
                                                   (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_tx_arrivals");
+    return MACH_OK;
+}
+
+int mach_analytic_signal(const float *rf, uint64_t n_samples, uint64_t n_rx, uint64_t n_tx, uint64_t n_frames,
+                         void *out, int32_t out_type, void *stream) {
+    g_last_error.clear();
+    if (out_type != MACH_DATA_C64 && out_type != MACH_DATA_C32H)
+        return fail(MACH_ERR_UNSUPPORTED, "Error: analytic signal output must be MACH_DATA_C64 or MACH_DATA_C32H");
+    if (n_samples == 0 || n_rx == 0 || n_tx == 0 || n_frames == 0) return MACH_OK;
+    mach::HilbertPlan plan;
+    if (!mach::hilbert_plan((int64_t)n_samples, &plan))
+        return fail(MACH_ERR_UNSUPPORTED, "Error: n_samples = " + std::to_string(n_samples) +
+                                              " is not a product of 2, 3 and 5 up to " +
+                                              std::to_string(mach::kHilbertMaxSamples));
+    if (rf == nullptr || out == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: rf / out is NULL");
+    if (n_rx > (uint64_t)INT32_MAX || n_tx > (uint64_t)INT32_MAX || n_frames > (uint64_t)INT32_MAX ||
+        (long double)n_samples * n_rx * n_tx * n_frames > (long double)INT64_MAX / 8)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: array dimensions exceed the supported range");
+    const hipError_t e = mach::launch_analytic_signal(rf, (int64_t)n_samples, (int64_t)n_rx, (int64_t)n_tx,
+                                                      (int64_t)n_frames, out, out_type == MACH_DATA_C32H,
+                                                      (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mach_analytic_signal");
     return MACH_OK;
 }
 

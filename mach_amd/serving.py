@@ -283,7 +283,7 @@ def _fusion_groups(reqs, fuse=True, max_frames=256):
 
 
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
-                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 20):
+                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22):
     """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
     fused into one ensemble (``fuse``); a request or fused group of at least ``plain_min_voxel_frames``
     voxel-frames runs as its own plain launch within the batch (``None``: never). Returns a

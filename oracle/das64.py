@@ -41,6 +41,12 @@ distance per ``src/mach/wavefront.py:8-50`` (plane: (p - o) . d) and ``53-116`` 
 coordinates with an explicit summation order ((x + y) + z), as the reference's callers do on the host
 (``src/mach/io/uff.py:37-67, 156-158``).
 
+RF -> IQ preprocessing (``analytic_signal64``): the analytic signal along the sample axis as
+``scipy.signal.hilbert`` computes it (FFT, keep DC and the even-length Nyquist bin, double the
+positive frequencies, zero the negative ones, inverse FFT), then the rearrangement from
+(n_samples, n_elements, n_waves, n_frames) to (n_waves, n_elements, n_samples, n_frames) of the
+reference's UFF preprocessing (``src/mach/io/uff.py:70-103``).
+
 Two edge decisions shared with the HIP kernel (the reference's behaviour there is a bug or undefined):
 
 * nearest neighbour at s in (n-1, n-0.5] rounds to n when n is even (``kernel.cu:117-122`` reads one
@@ -386,3 +392,21 @@ def wave_arrivals64(waves, scan) -> np.ndarray:
             d = origin_focus * np.sign(v[2] - o[2]) - focus_point * np.sign(v[2] - z)
         out.append((d / float(c) + float(offset)).astype(np.float32))
     return np.stack(out) if out else np.zeros((0, len(p)), dtype=np.float32)
+
+
+def analytic_signal64(rf) -> np.ndarray:
+    """[S, E, T(, F)] real -> [T, E, S, F] complex128 analytic signal along S (``io/uff.py:70-103``)."""
+    x = np.asarray(rf, dtype=np.float64)
+    if x.ndim == 3:
+        x = x[..., None]
+    n = x.shape[0]
+    X = np.fft.fft(x, axis=0)
+    h = np.zeros(n)
+    if n % 2 == 0:
+        h[0] = h[n // 2] = 1.0
+        h[1:n // 2] = 2.0
+    else:
+        h[0] = 1.0
+        h[1:(n + 1) // 2] = 2.0
+    xa = np.fft.ifft(X * h[:, None, None, None], axis=0)
+    return np.ascontiguousarray(xa.transpose(2, 1, 0, 3))

@@ -54,3 +54,30 @@ def test_db_zero_normalises_to_the_maximum():
     assert d.max() == 0.0
     # all-zero image: every value is epsilon, so the image is 0 dB everywhere
     np.testing.assert_array_equal(db_zero64(np.zeros(4)), np.zeros(4))
+
+
+def test_analytic_signal_matches_scipy_hilbert_and_rearranges():
+    """analytic_signal64 = scipy.signal.hilbert along samples (the reference's UFF preprocessing,
+    src/mach/io/uff.py:86-88), then (S, E, T, F) -> (T, E, S, F)."""
+    import scipy.signal
+
+    from oracle.das64 import analytic_signal64
+
+    rng = np.random.default_rng(3)
+    for n in (9, 10, 64, 96):
+        rf = rng.normal(size=(n, 3, 2, 4))
+        got = analytic_signal64(rf)
+        ref = scipy.signal.hilbert(rf, axis=0).transpose(2, 1, 0, 3)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+        assert got.shape == (2, 3, n, 4)
+        np.testing.assert_allclose(got.real, rf.transpose(2, 1, 0, 3), atol=1e-12)  # real part is the RF
+
+
+def test_analytic_signal_of_a_cosine_is_the_complex_exponential():
+    # a bin-centred cosine has the exact analytic signal exp(j w t); a wrong sign or a dropped doubling fails
+    from oracle.das64 import analytic_signal64
+
+    n, k = 64, 5
+    t = np.arange(n)
+    rf = np.cos(2 * np.pi * k * t / n)[:, None, None]
+    np.testing.assert_allclose(analytic_signal64(rf)[0, 0, :, 0], np.exp(1j * 2 * np.pi * k * t / n), atol=1e-12)
