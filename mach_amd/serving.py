@@ -18,6 +18,9 @@ server:
   cannot make. Either way nothing synchronises per request. Up to two batches are in flight, on two
   streams, so requests that arrive while a batch runs are admitted and start while the previous
   batch drains;
+* while the GPU is busy, holds large requests (``hold_min_points``) back for up to ``max_hold_s`` so
+  that same-geometry requests arriving meanwhile fuse with them; requests therefore complete out of
+  admission order, and the rings release slots in any order;
 * reports per-request latency (arrival -> result ready on the device), p50/p99, and throughput in
   points/s over the makespan.
 
@@ -82,9 +85,10 @@ class ServeRequest:
 
 
 class DeviceRing:
-    """First-in first-out byte ring on the device. Allocations are 256-byte aligned and contiguous
-    (an allocation that would cross the end starts again at 0); frees happen in allocation order,
-    which is the order the server completes requests in."""
+    """Byte ring on the device. Allocations are 256-byte aligned and contiguous (an allocation that
+    would cross the end starts again at 0). Allocations may be released in any order (``free``);
+    space is reclaimed from the oldest end, so a released slot behind a live older one stays occupied
+    until that one is released too."""
 
     ALIGN = 256
 
@@ -92,12 +96,13 @@ class DeviceRing:
         self.capacity = int(capacity_bytes) // self.ALIGN * self.ALIGN
         self.buf = torch.empty(self.capacity, dtype=torch.uint8, device=device)
         self.head = 0
-        self.live = collections.deque()  # (start, size) in allocation order
+        self.live = collections.deque()  # [start, size, released] in allocation order
         self._undo = None
 
     @property
     def used(self) -> int:
-        return sum(n for _, n in self.live)
+        """Bytes not yet reclaimable (live allocations and released ones behind a live older one)."""
+        return sum(e[1] for e in self.live)
 
     def alloc(self, nbytes: int):
         """Reserve ``nbytes``; returns (start, size) or None if it does not fit now."""
@@ -108,7 +113,7 @@ class DeviceRing:
             start = 0
         else:
             tail = self.live[0][0]
-            if self.head > tail:  # live bytes occupy [tail, head)
+            if self.head > tail:  # occupied bytes are [tail, head)
                 if self.head + n <= self.capacity:
                     start = self.head
                 elif n <= tail:
@@ -120,7 +125,7 @@ class DeviceRing:
             else:
                 return None
         self._undo = self.head
-        self.live.append((start, n))
+        self.live.append([start, n, False])
         self.head = start + n
         return (start, n)
 
@@ -129,10 +134,25 @@ class DeviceRing:
         self.live.pop()
         self.head = self._undo if self.live else 0
 
-    def free_oldest(self):
-        self.live.popleft()
+    def _reclaim(self):
+        while self.live and self.live[0][2]:
+            self.live.popleft()
         if not self.live:
             self.head = 0
+
+    def free(self, slot):
+        """Release the allocation ``slot`` (as returned by ``alloc``), in any order."""
+        for e in self.live:
+            if e[0] == slot[0] and e[1] == slot[1] and not e[2]:
+                e[2] = True
+                break
+        else:
+            raise KeyError(f"slot {slot} is not allocated")
+        self._reclaim()
+
+    def free_oldest(self):
+        self.live[0][2] = True
+        self._reclaim()
 
     def view(self, slot, dtype, shape):
         start, _ = slot
@@ -282,12 +302,37 @@ def _fusion_groups(reqs, fuse=True, max_frames=256):
     return groups
 
 
+def _select(admitted, gpu_busy, now, hold_min_points, max_hold_s, max_fused_frames):
+    """Requests to launch now. With the GPU idle, everything admitted. Otherwise large requests
+    (``hold_min_points``) that share a geometry are held back so that partners arriving meanwhile fuse
+    with them: of those, only the largest group, groups that already fill ``max_fused_frames`` and
+    groups whose oldest member has waited ``max_hold_s`` go now."""
+    if not gpu_busy or hold_min_points is None:
+        return admitted, []
+    groups = collections.OrderedDict()
+    for item in admitted:
+        r = item[0]
+        if r.points >= hold_min_points:
+            groups.setdefault((id(r.geometry), r.source.dtype, tuple(r.source.shape[:3])), []).append(item)
+    if not groups:
+        return admitted, []
+    frames = {k: sum(it[0].source.shape[3] for it in g) for k, g in groups.items()}
+    largest = max(groups, key=lambda k: frames[k])
+    go_keys = {k for k, g in groups.items()
+               if k == largest or frames[k] >= max_fused_frames or now - min(it[0].admit_s for it in g) >= max_hold_s}
+    held = {id(it) for k, g in groups.items() if k not in go_keys for it in g}
+    return [it for it in admitted if id(it) not in held], [it for it in admitted if id(it) in held]
+
+
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
-                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22):
+                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22, hold_min_points=4e9,
+                  max_hold_s=0.5):
     """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
     fused into one ensemble (``fuse``); a request or fused group of at least ``plain_min_voxel_frames``
-    voxel-frames runs as its own plain launch within the batch (``None``: never). Returns a
-    ServeReport; with ``keep_outputs`` each request's ``out`` is a private copy of its result."""
+    voxel-frames runs as its own plain launch within the batch (``None``: never). With
+    ``hold_min_points``, large requests wait (at most ``max_hold_s`` while the GPU is busy) for
+    same-geometry partners to fuse with. Returns a ServeReport; with ``keep_outputs`` each request's
+    ``out`` is a private copy of its result."""
     free, _total = torch.cuda.mem_get_info(device)
     budget = min(int(budget_bytes), int(free * 0.8))
     data_ring = DeviceRing(budget * 3 // 4, device)
@@ -332,13 +377,17 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
                 if keep_outputs:  # private copy, complete before the ring slot can be reused
                     r.out = r.out.clone()
                     torch.cuda.current_stream(device).synchronize()
-                data_ring.free_oldest()
-                out_ring.free_oldest()
+                data_ring.free(r.data_slot)
+                out_ring.free(r.out_slot)
             n_done += len(done_reqs)
-        # launch: everything admitted goes into the next batch
+        # launch: what is admitted (less the large requests held for fusion partners) forms a batch
+        take = []
         if admitted and len(inflight) < 2:
-            take = admitted[:max_batch]
-            del admitted[:max_batch]
+            go, admitted = _select(admitted, len(inflight) > 0, time.perf_counter() - t0, hold_min_points,
+                                   max_hold_s, max_fused_frames)
+            take = go[:max_batch]
+            admitted = go[max_batch:] + admitted
+        if take:
             stream = compute[n_batches % 2]
             for _r, ev in take:
                 stream.wait_event(ev)
@@ -367,10 +416,10 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
                     n_direct += len(group)
                 else:
                     breqs.append(req)
+            if breqs:  # small requests first: they finish sooner
+                runner.submit(breqs, stream=stream)
             for req in direct:  # GPU-filling requests: their own launch, tuned layout, balanced remap
                 _batch.run_plain(req, stream=stream)
-            if breqs:
-                runner.submit(breqs, stream=stream)
             ev = torch.cuda.Event()
             ev.record(stream)
             inflight.append(([r for r, _ in take], ev, keep))

@@ -30,17 +30,20 @@ def _reset(reqs):
         r.out = None
 
 
-@pytest.mark.parametrize("fuse,direct", [(True, None), (False, None), (True, 1 << 14)])
-def test_mix_equals_alone(device, burst_reqs, fuse, direct):
-    """Each result is bitwise the same alone and in the mix: with and without fusion, and with large
-    requests launched as their own plain kernels inside a batch (``direct`` voxel-frames threshold)."""
+@pytest.mark.parametrize("fuse,direct,hold", [(True, None, None), (False, None, None), (True, 1 << 14, None),
+                                               (True, 1 << 14, 1)])
+def test_mix_equals_alone(device, burst_reqs, fuse, direct, hold):
+    """Each result is bitwise the same alone and in the mix: with and without fusion, with large
+    requests launched as their own plain kernels inside a batch (``direct`` voxel-frames threshold),
+    and with requests held back for fusion partners (``hold`` points threshold; out-of-order
+    completion and ring release)."""
     reqs = burst_reqs
     _reset(reqs)
     plain = serving.serve_plain(reqs, device, keep_outputs=True)
     alone = {r.rid: r.out for r in reqs}
     _reset(reqs)
     mixed = serving.serve_batched(reqs, device, budget_bytes=8 << 30, keep_outputs=True, fuse=fuse,
-                                  plain_min_voxel_frames=direct)
+                                  plain_min_voxel_frames=direct, hold_min_points=hold)
     for r in reqs:
         assert torch.equal(r.out, alone[r.rid]), r.rid
     print(plain.as_dict(), mixed.as_dict(), flush=True)

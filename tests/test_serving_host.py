@@ -1,4 +1,5 @@
-"""Host logic of the serving loop that needs no GPU: the FIFO device ring allocator."""
+"""Host logic of the serving loop that needs no GPU: the device ring allocator, request fusion and
+the batch selection that holds large requests for fusion partners."""
 
 import random
 
@@ -58,3 +59,51 @@ def test_fusion_groups_by_geometry_and_frame_cap():
     groups = [[r.rid for r in g] for g in _fusion_groups(rs, True, 256)]
     assert groups == [[0, 2], [1, 4], [3], [5]]  # frame cap 256, dtype separates
     assert [[r.rid for r in g] for g in _fusion_groups(rs, False)] == [[i] for i in range(6)]
+
+
+def test_ring_out_of_order_frees_never_overlap():
+    """Releases in any order; space is reclaimed from the oldest end only."""
+    ring = DeviceRing(256 * 64, "cpu")
+    rng = random.Random(1)
+    live = []  # slots not yet released
+    for _ in range(20000):
+        if live and rng.random() < 0.5:
+            slot = live.pop(rng.randrange(len(live)))
+            ring.free(slot)
+        else:
+            slot = ring.alloc(rng.randint(1, 12) * 256)
+            if slot is None:
+                assert live or ring.used > 0
+                continue
+            for a, n in live:
+                assert slot[0] + slot[1] <= a or a + n <= slot[0]
+            assert slot[0] + slot[1] <= ring.capacity
+            live.append(slot)
+        assert ring.used >= sum(n for _, n in live)
+    for slot in list(live):
+        ring.free(slot)
+    assert ring.used == 0 and ring.alloc(ring.capacity) == (0, ring.capacity)
+    with pytest.raises(KeyError):
+        ring.free((0, ring.capacity + 256))
+
+
+def test_select_holds_large_requests_for_partners():
+    import torch
+
+    from mach_amd.serving import ServeRequest, _select
+
+    ga, gb, gc = object(), object(), object()
+
+    def item(rid, geo, points, admit=0.0, frames=8):
+        r = ServeRequest(rid, 0.0, geo, torch.empty((1, 4, 8, frames)), "x", points)
+        r.admit_s = admit
+        return (r, None)
+
+    adm = [item(0, ga, 10), item(1, gb, 100), item(2, gc, 100), item(3, gb, 100), item(4, ga, 10)]
+    go, held = _select(adm, gpu_busy=False, now=0.0, hold_min_points=50, max_hold_s=1.0, max_fused_frames=256)
+    assert [it[0].rid for it in go] == [0, 1, 2, 3, 4] and held == []  # an idle GPU takes everything
+    go, held = _select(adm, gpu_busy=True, now=0.1, hold_min_points=50, max_hold_s=1.0, max_fused_frames=256)
+    assert [it[0].rid for it in go] == [0, 1, 3, 4]  # small ones and the largest large group (gb)
+    assert [it[0].rid for it in held] == [2]
+    go, held = _select(adm, gpu_busy=True, now=2.0, hold_min_points=50, max_hold_s=1.0, max_fused_frames=256)
+    assert held == []  # waited long enough
