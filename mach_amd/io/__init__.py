@@ -1,0 +1,5 @@
+"""Readers of acquisition files that exist locally (the reference's ``mach.io``; nothing here downloads)."""
+
+from . import must
+
+__all__ = ["must"]
