@@ -130,6 +130,10 @@ def _check_writable(out):
         raise ValueError("Output array `out` is not writable.")
 
 
+# MACH_AMD_SYNC=1 restores the reference's synchronise-after-every-call behaviour (for timing a call the
+# way the reference's benchmarks do); by default calls are ordered on the current stream only.
+_SYNC_EACH_CALL = os.environ.get("MACH_AMD_SYNC", "0") == "1"
+
 # Voxels below this count are processed in the caller's order (no reordering benefit).
 _ORDER_MIN_VOXELS = 4096
 _ORDER_CACHE: "collections.OrderedDict" = collections.OrderedDict()
@@ -241,6 +245,8 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
         _lib.check(status)
         if host_out:
             out.copy_(dev_out.cpu())
+        elif _SYNC_EACH_CALL:  # the reference synchronises the device after every call (kernel.cu:874)
+            torch.cuda.current_stream(device).synchronize()
 
 
 def beamform(
