@@ -163,14 +163,16 @@ __device__ __forceinline__ int bcast(int x, int j, int lane) {
     }
 }
 
-// Loads FPL consecutive frames of one sample row. `off` is the element offset of the first frame.
+// Bytes per stored channel-data element.
+template <int ST>
+constexpr int kElemBytes = ST == kStoreC64 ? 8 : 4;
+
+// Loads FPL consecutive frames starting at byte address q.
 template <int ST, int FPL>
-__device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t off, float (&re)[FPL],
-                                         float (&im)[FPL]) {
+__device__ __forceinline__ void load_at(const char *__restrict__ q, float (&re)[FPL], float (&im)[FPL]) {
     if constexpr (ST == kStoreF32) {
-        const float *q = (const float *)base + off;
         if constexpr (FPL == 1) {
-            re[0] = q[0];
+            re[0] = *(const float *)q;
         } else {
             const fvec<FPL> v = *(const fvec<FPL> *)q;
 #pragma unroll
@@ -179,7 +181,6 @@ __device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t 
 #pragma unroll
         for (int k = 0; k < FPL; ++k) im[k] = 0.f;
     } else if constexpr (ST == kStoreC64) {
-        const float *q = (const float *)base + 2 * off;
         if constexpr (FPL == 1) {
             const fvec<2> v = *(const fvec<2> *)q;
             re[0] = v[0];
@@ -189,12 +190,11 @@ __device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t 
             re[0] = v[0]; im[0] = v[1]; re[1] = v[2]; im[1] = v[3];
         } else {
             const fvec<4> a = *(const fvec<4> *)q;
-            const fvec<4> b = *(const fvec<4> *)(q + 4);
+            const fvec<4> b = *(const fvec<4> *)(q + 16);
             re[0] = a[0]; im[0] = a[1]; re[1] = a[2]; im[1] = a[3];
             re[2] = b[0]; im[2] = b[1]; re[3] = b[2]; im[3] = b[3];
         }
     } else {
-        const _Float16 *q = (const _Float16 *)base + 2 * off;
         const hvec<2 * FPL> v = *(const hvec<2 * FPL> *)q;
 #pragma unroll
         for (int k = 0; k < FPL; ++k) {
@@ -204,18 +204,37 @@ __device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t 
     }
 }
 
+// Loads FPL consecutive frames of one sample row. `off` is the element offset of the first frame.
+template <int ST, int FPL>
+__device__ __forceinline__ void load_row(const void *__restrict__ base, int64_t off, float (&re)[FPL],
+                                         float (&im)[FPL]) {
+    load_at<ST, FPL>((const char *)base + off * kElemBytes<ST>, re, im);
+}
+
+// 3-point Lagrange weights about the centre tap for x = s - round(s) (kernel.cu:233-240).
+struct QuadWeights {
+    float l0, l1, l2;
+};
+
+__device__ __forceinline__ QuadWeights quad_weights(float x) {
+    return QuadWeights{0.5f * x * (x - 1.f), (1.f - x) * (1.f + x), 0.5f * x * (x + 1.f)};
+}
+
+// Interpolates the taps at byte addresses q0, q1, q2 (consecutive samples; q1, q2 unused by nearest
+// neighbour, q2 by linear) and adds w * exp(j phase) * value to the accumulators. `frac` is the lerp
+// fraction (linear); quadratic interpolation takes its weights `qw`.
 template <int ST, int INTERP, int FPL>
-__device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, int64_t off, int64_t step,
-                                                float frac, float wc, float ws, float (&acc_re)[FPL],
-                                                float (&acc_im)[FPL]) {
+__device__ __forceinline__ void interp_accumulate(const char *q0, const char *q1, const char *q2, float frac,
+                                                  QuadWeights qw, float wc, float ws, float (&acc_re)[FPL],
+                                                  float (&acc_im)[FPL]) {
     constexpr bool kComplex = ST != kStoreF32;
     float vr[FPL], vi[FPL];
     if constexpr (INTERP == kNearest) {
-        load_row<ST, FPL>(data, off, vr, vi);
+        load_at<ST, FPL>(q0, vr, vi);
     } else if constexpr (INTERP == kLinear) {
         float ar[FPL], ai[FPL], br[FPL], bi[FPL];
-        load_row<ST, FPL>(data, off, ar, ai);
-        load_row<ST, FPL>(data, off + step, br, bi);
+        load_at<ST, FPL>(q0, ar, ai);
+        load_at<ST, FPL>(q1, br, bi);
 #pragma unroll
         for (int k = 0; k < FPL; ++k) {  // lerp a + t (b - a) (vendor/helper_math.h:1150-1152)
             vr[k] = fmaf(frac, br[k] - ar[k], ar[k]);
@@ -223,17 +242,13 @@ __device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, i
         }
     } else {
         float ar[FPL], ai[FPL], br[FPL], bi[FPL], cr[FPL], ci[FPL];
-        load_row<ST, FPL>(data, off, ar, ai);
-        load_row<ST, FPL>(data, off + step, br, bi);
-        load_row<ST, FPL>(data, off + 2 * step, cr, ci);
-        const float x = frac;  // Lagrange weights about the centre tap (kernel.cu:233-240)
-        const float l0 = 0.5f * x * (x - 1.f);
-        const float l1 = (1.f - x) * (1.f + x);
-        const float l2 = 0.5f * x * (x + 1.f);
+        load_at<ST, FPL>(q0, ar, ai);
+        load_at<ST, FPL>(q1, br, bi);
+        load_at<ST, FPL>(q2, cr, ci);
 #pragma unroll
         for (int k = 0; k < FPL; ++k) {
-            vr[k] = fmaf(l2, cr[k], fmaf(l1, br[k], l0 * ar[k]));
-            if constexpr (kComplex) vi[k] = fmaf(l2, ci[k], fmaf(l1, bi[k], l0 * ai[k]));
+            vr[k] = fmaf(qw.l2, cr[k], fmaf(qw.l1, br[k], qw.l0 * ar[k]));
+            if constexpr (kComplex) vi[k] = fmaf(qw.l2, ci[k], fmaf(qw.l1, bi[k], qw.l0 * ai[k]));
         }
     }
 #pragma unroll
@@ -245,6 +260,17 @@ __device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, i
             acc_re[k] = fmaf(wc, vr[k], acc_re[k]);
         }
     }
+}
+
+// interp_accumulate at element offset `off` of `data`, taps `step` elements apart.
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void accumulate_pair(const void *__restrict__ data, int64_t off, int64_t step,
+                                                float frac, float wc, float ws, float (&acc_re)[FPL],
+                                                float (&acc_im)[FPL]) {
+    constexpr int kE = kElemBytes<ST>;
+    const char *q0 = (const char *)data + off * kE;
+    const QuadWeights qw = INTERP == kQuadratic ? quad_weights(frac) : QuadWeights{0.f, 0.f, 0.f};
+    interp_accumulate<ST, INTERP, FPL>(q0, q0 + step * kE, q0 + 2 * step * kE, frac, qw, wc, ws, acc_re, acc_im);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (linear id i runs on XCD i % 8), each XCD with
@@ -300,7 +326,10 @@ __device__ __forceinline__ void peak_max(float *peak, float x) {
 //                  lanes of a voxel are then combined by an xor butterfly, so the result is
 //                  deterministic (and the same for every CH).
 // With p.peak set, the maximum value written is folded into *peak.
-template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1>
+// WIDE selects 64-bit element offsets for channel blocks of 4 GiB or more (p.bytes32 == 0); otherwise
+// each tap is a wave-uniform 64-bit row base (scalar registers) plus one 32-bit per-lane byte offset,
+// which the loads take directly (scalar-base + vector-offset form, no 64-bit address arithmetic).
+template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false>
 __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
     constexpr int LV = 64 / LF;
     constexpr int kGroup = LF * FPL;  // frames per group across the LF lanes of a voxel
@@ -365,6 +394,12 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                     s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
                     if (tt < nt) s[tt] = pair_time<INTERP>(p, g, Tv[tt]);
                 }
+                // Single-transmit quadratic launches: the setup lane forms the Lagrange weights once per
+                // pair and broadcasts them, instead of every lane of the voxel forming them per element
+                // (the same float operations, so the same values).
+                constexpr bool kPreWeights = INTERP == kQuadratic && TC == 1;
+                QuadWeights pw_own{0.f, 0.f, 0.f};
+                if constexpr (kPreWeights) pw_own = quad_weights(s[0].frac);
                 // Only elements that some voxel of this wave uses under some transmit of the chunk:
                 // fold the wave's ballot (bit lv * LF + j: lane lv * LF + j computed element e0 + j of
                 // voxel lv) onto the LF element slots and walk the set bits with a scalar loop. The
@@ -388,19 +423,46 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                     for (int tt = 0; tt < TC; ++tt) {
                         if (tt >= nt) break;
                         const int i0 = bcast<LF>(s[tt].i0, j, lane);
-                        const float frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
+                        float frac = 0.f;
+                        QuadWeights qw{0.f, 0.f, 0.f};
+                        if constexpr (kPreWeights) {
+                            qw.l0 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l0), j, lane));
+                            qw.l1 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l1), j, lane));
+                            qw.l2 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l2), j, lane));
+                        } else if constexpr (INTERP != kNearest) {
+                            frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
+                            if constexpr (INTERP == kQuadratic) qw = quad_weights(frac);
+                        }
                         const float wc = __int_as_float(bcast<LF>(__float_as_int(s[tt].wc), j, lane));
                         const float ws = __int_as_float(bcast<LF>(__float_as_int(s[tt].ws), j, lane));
                         if (i0 >= 0 && fok[0]) {
+                            // first element of the channel's [sample][frame] block (wave-uniform)
                             const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;
-                            const int64_t off = row0 + f0 +
-                                                (p.rows32 ? (int64_t)((uint32_t)i0 * (uint32_t)p.n_frames)
-                                                          : (int64_t)i0 * p.n_frames);
+                            if constexpr (!WIDE) {
+                                constexpr int kE = kElemBytes<ST>;
+                                const char *r0 = (const char *)p.data + row0 * kE;
+                                const char *r1 = r0 + p.tap_step * kE;
+                                const char *r2 = r1 + p.tap_step * kE;
+                                const uint32_t o = ((uint32_t)i0 * (uint32_t)p.n_frames + (uint32_t)f0) * (uint32_t)kE;
 #pragma unroll
-                            for (int c = 0; c < CH; ++c) {
-                                if (c == 0 || fok[c])
-                                    accumulate_pair<ST, INTERP, FPL>(p.data, off + (int64_t)c * kGroup, p.tap_step,
-                                                                     frac, wc, ws, acc_re[c], acc_im[c]);
+                                for (int c = 0; c < CH; ++c) {
+                                    const uint32_t oc = o + (uint32_t)(c * kGroup * kE);
+                                    if (c == 0 || fok[c])
+                                        interp_accumulate<ST, INTERP, FPL>(r0 + oc, r1 + oc, r2 + oc, frac, qw, wc,
+                                                                           ws, acc_re[c], acc_im[c]);
+                                }
+                            } else {
+                                const int64_t off = row0 + f0 + (int64_t)i0 * p.n_frames;
+#pragma unroll
+                                for (int c = 0; c < CH; ++c) {
+                                    if (c == 0 || fok[c]) {
+                                        const char *q0 = (const char *)p.data + (off + (int64_t)c * kGroup) *
+                                                                                    kElemBytes<ST>;
+                                        const int64_t sb = p.tap_step * kElemBytes<ST>;
+                                        interp_accumulate<ST, INTERP, FPL>(q0, q0 + sb, q0 + 2 * sb, frac, qw, wc, ws,
+                                                                           acc_re[c], acc_im[c]);
+                                    }
+                                }
                             }
                         }
                     }

@@ -50,7 +50,8 @@ struct DasParams {
     int32_t rotate;          // IQ data with fmod != 0
     int32_t accumulate;      // out += result (1) or out = result (0)
     int32_t apod;            // alpha > 0
-    int32_t rows32;          // n_samples * n_frames < 2^32: sample offsets within a channel fit 32 bits
+    int32_t bytes32;         // n_samples * n_frames * element bytes < 2^32: byte offsets within a
+                             // channel's block fit 32 bits
     int32_t output;          // OutputKind
     int32_t xcd_group;       // block-to-XCD remap (xcd_remap): 0 = contiguous eighths, > 0 = runs of this
                              // many blocks; < 0 = chosen per launch (xcd_group_for)

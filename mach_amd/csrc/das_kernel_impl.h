@@ -14,13 +14,13 @@ namespace {
 
 using namespace dev;
 
-template <int ST, int INTERP, int LF, int FPL, int TC, int CH>
+template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false>
 __global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    das_block<ST, INTERP, LF, FPL, TC, CH>(p, lb % gridDim.x, lb / gridDim.x);
+    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE>(p, lb % gridDim.x, lb / gridDim.x);
 }
 
-template <int ST, int INTERP, int LF, int FPL, int CH>
+template <int ST, int INTERP, int LF, int FPL, int CH, bool WIDE = false>
 hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     constexpr int LV = 64 / LF;
     const int64_t n_vgroups = (p.n_vox + LV - 1) / LV;
@@ -33,9 +33,9 @@ hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     DasParams q = p;
     q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
     if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
-        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH>), grid, dim3(256, 1, 1), 0, stream, q);
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE>), grid, dim3(256, 1, 1), 0, stream, q);
     } else {
-        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, 1, CH>), grid, dim3(256, 1, 1), 0, stream, q);
+        hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, 1, CH, WIDE>), grid, dim3(256, 1, 1), 0, stream, q);
     }
     return hipGetLastError();
 }
@@ -84,11 +84,23 @@ hipError_t dispatch_interp(const DasParams &p, int interp, DasLayout l, hipStrea
     }
 }
 
+// Channel blocks of 4 GiB or more (64-bit offsets inside a channel) run one layout, 64 lanes x 1
+// frame, which is valid for any frame count and alignment and gives the same sums as every other.
+template <int ST>
+hipError_t launch_wide(const DasParams &p, int interp, hipStream_t stream) {
+    switch (interp) {
+        case kNearest: return launch_one<ST, kNearest, 64, 1, 1, true>(p, stream);
+        case kQuadratic: return launch_one<ST, kQuadratic, 64, 1, 1, true>(p, stream);
+        default: return launch_one<ST, kLinear, 64, 1, 1, true>(p, stream);
+    }
+}
+
 }  // namespace
 
 // Entry of one storage type's translation unit.
 template <int ST>
 hipError_t launch_das_store(const DasParams &p, int interp, DasLayout l, hipStream_t stream) {
+    if (!p.bytes32) return launch_wide<ST>(p, interp, stream);
     return dispatch_interp<ST>(p, interp, l, stream);
 }
 

@@ -117,7 +117,8 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->rotate = (a->data_type != MACH_DATA_F32 && a->modulation_freq_hz != 0.0f) ? 1 : 0;
     p->accumulate = a->accumulate ? 1 : 0;
     p->apod = a->tukey_alpha > 0.0f ? 1 : 0;  // kernel.cu:749
-    p->rows32 = (a->n_samples * a->n_frames) < (1ull << 32) ? 1 : 0;
+    const uint64_t elem = a->data_type == MACH_DATA_C64 ? 8u : 4u;
+    p->bytes32 = ((long double)a->n_samples * a->n_frames * elem) < 4294967296.0L ? 1 : 0;
     p->output = a->output;
     p->peak = a->peak;
     p->xcd_group = xcd_group_override();
@@ -231,6 +232,9 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
         int st = validate_and_build(&requests[i], &params[i]);
         if (st == MACH_OK && params[i].output != mach::kOutSignal)
             st = fail(MACH_ERR_UNSUPPORTED, "Error: batches run the signal output only");
+        if (st == MACH_OK && !params[i].bytes32)
+            st = fail(MACH_ERR_UNSUPPORTED, "Error: channel blocks of 4 GiB or more (n_samples * n_frames * element "
+                                            "bytes) run through mach_beamform only");
         Plan plan{false, 0, mach::DasLayout{1, 1}};
         if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0) st = resolve_plan(&requests[i], params[i], &plan);
         if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
