@@ -220,35 +220,94 @@ __device__ __forceinline__ QuadWeights quad_weights(float x) {
     return QuadWeights{0.5f * x * (x - 1.f), (1.f - x) * (1.f + x), 0.5f * x * (x + 1.f)};
 }
 
-// Interpolates the taps at byte addresses q0, q1, q2 (consecutive samples; q1, q2 unused by nearest
-// neighbour, q2 by linear) and adds w * exp(j phase) * value to the accumulators. `frac` is the lerp
+// One element's pair setup as the lanes of a voxel receive it from the lane that computed it (slot j of
+// the voxel's LF lanes). With kPre the setup lane has formed the quadratic weights `pw` itself.
+struct ElemCoef {
+    int i0;
+    float frac;
+    QuadWeights qw;
+    float wc, ws;
+};
+
+template <int LF, int INTERP, bool kPre>
+__device__ __forceinline__ ElemCoef elem_coef(const PairSetup &s, QuadWeights pw, int j, int lane) {
+    ElemCoef c{bcast<LF>(s.i0, j, lane), 0.f, {0.f, 0.f, 0.f}, 0.f, 0.f};
+    if constexpr (kPre) {
+        c.qw.l0 = __int_as_float(bcast<LF>(__float_as_int(pw.l0), j, lane));
+        c.qw.l1 = __int_as_float(bcast<LF>(__float_as_int(pw.l1), j, lane));
+        c.qw.l2 = __int_as_float(bcast<LF>(__float_as_int(pw.l2), j, lane));
+    } else if constexpr (INTERP != kNearest) {
+        c.frac = __int_as_float(bcast<LF>(__float_as_int(s.frac), j, lane));
+        if constexpr (INTERP == kQuadratic) c.qw = quad_weights(c.frac);
+    }
+    c.wc = __int_as_float(bcast<LF>(__float_as_int(s.wc), j, lane));
+    c.ws = __int_as_float(bcast<LF>(__float_as_int(s.ws), j, lane));
+    return c;
+}
+
+// The taps of one (element, frame group) as loaded: FP16 pairs stay packed until they are combined, so
+// taps loaded ahead of their use hold half the registers.
+template <int ST, int INTERP, int FPL>
+struct Taps {
+    static constexpr int kN = INTERP == kNearest ? 1 : (INTERP == kLinear ? 2 : 3);
+    hvec<2 * FPL> h[kN];             // FP16 storage
+    float re[kN][FPL], im[kN][FPL];  // float32 / complex64 storage
+};
+
+// Loads the taps at byte addresses q0, q1, q2 (consecutive samples; q1, q2 unused by nearest
+// neighbour, q2 by linear).
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void load_taps(const char *q0, const char *q1, const char *q2,
+                                          Taps<ST, INTERP, FPL> &t) {
+    constexpr int kN = Taps<ST, INTERP, FPL>::kN;
+    const char *q[3] = {q0, q1, q2};
+#pragma unroll
+    for (int n = 0; n < kN; ++n) {
+        if constexpr (ST == kStoreC32H)
+            t.h[n] = *(const hvec<2 * FPL> *)q[n];
+        else
+            load_at<ST, FPL>(q[n], t.re[n], t.im[n]);
+    }
+}
+
+// Interpolates loaded taps and adds w * exp(j phase) * value to the accumulators. `frac` is the lerp
 // fraction (linear); quadratic interpolation takes its weights `qw`.
 template <int ST, int INTERP, int FPL>
-__device__ __forceinline__ void interp_accumulate(const char *q0, const char *q1, const char *q2, float frac,
-                                                  QuadWeights qw, float wc, float ws, float (&acc_re)[FPL],
-                                                  float (&acc_im)[FPL]) {
+__device__ __forceinline__ void combine_taps(const Taps<ST, INTERP, FPL> &t, float frac, QuadWeights qw, float wc,
+                                             float ws, float (&acc_re)[FPL], float (&acc_im)[FPL]) {
     constexpr bool kComplex = ST != kStoreF32;
-    float vr[FPL], vi[FPL];
-    if constexpr (INTERP == kNearest) {
-        load_at<ST, FPL>(q0, vr, vi);
-    } else if constexpr (INTERP == kLinear) {
-        float ar[FPL], ai[FPL], br[FPL], bi[FPL];
-        load_at<ST, FPL>(q0, ar, ai);
-        load_at<ST, FPL>(q1, br, bi);
+    constexpr int kN = Taps<ST, INTERP, FPL>::kN;
+    float r[kN][FPL], i[kN][FPL];
 #pragma unroll
-        for (int k = 0; k < FPL; ++k) {  // lerp a + t (b - a) (vendor/helper_math.h:1150-1152)
-            vr[k] = fmaf(frac, br[k] - ar[k], ar[k]);
-            if constexpr (kComplex) vi[k] = fmaf(frac, bi[k] - ai[k], ai[k]);
-        }
-    } else {
-        float ar[FPL], ai[FPL], br[FPL], bi[FPL], cr[FPL], ci[FPL];
-        load_at<ST, FPL>(q0, ar, ai);
-        load_at<ST, FPL>(q1, br, bi);
-        load_at<ST, FPL>(q2, cr, ci);
+    for (int n = 0; n < kN; ++n)
 #pragma unroll
         for (int k = 0; k < FPL; ++k) {
-            vr[k] = fmaf(qw.l2, cr[k], fmaf(qw.l1, br[k], qw.l0 * ar[k]));
-            if constexpr (kComplex) vi[k] = fmaf(qw.l2, ci[k], fmaf(qw.l1, bi[k], qw.l0 * ai[k]));
+            if constexpr (ST == kStoreC32H) {
+                r[n][k] = (float)t.h[n][2 * k];
+                i[n][k] = (float)t.h[n][2 * k + 1];
+            } else {
+                r[n][k] = t.re[n][k];
+                i[n][k] = t.im[n][k];
+            }
+        }
+    float vr[FPL], vi[FPL];
+    if constexpr (INTERP == kNearest) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            vr[k] = r[0][k];
+            vi[k] = i[0][k];
+        }
+    } else if constexpr (INTERP == kLinear) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {  // lerp a + t (b - a) (vendor/helper_math.h:1150-1152)
+            vr[k] = fmaf(frac, r[1][k] - r[0][k], r[0][k]);
+            if constexpr (kComplex) vi[k] = fmaf(frac, i[1][k] - i[0][k], i[0][k]);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            vr[k] = fmaf(qw.l2, r[2][k], fmaf(qw.l1, r[1][k], qw.l0 * r[0][k]));
+            if constexpr (kComplex) vi[k] = fmaf(qw.l2, i[2][k], fmaf(qw.l1, i[1][k], qw.l0 * i[0][k]));
         }
     }
 #pragma unroll
@@ -260,6 +319,33 @@ __device__ __forceinline__ void interp_accumulate(const char *q0, const char *q1
             acc_re[k] = fmaf(wc, vr[k], acc_re[k]);
         }
     }
+}
+
+// An empty use of loaded taps (no instructions): it pins the loads ahead of this point.
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void keep_taps(const Taps<ST, INTERP, FPL> &t) {
+#pragma unroll
+    for (int n = 0; n < Taps<ST, INTERP, FPL>::kN; ++n) {
+        if constexpr (ST == kStoreC32H) {
+            asm volatile("" ::"v"(t.h[n]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) {
+                asm volatile("" ::"v"(t.re[n][k]));
+                if constexpr (ST == kStoreC64) asm volatile("" ::"v"(t.im[n][k]));
+            }
+        }
+    }
+}
+
+// load_taps then combine_taps.
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void interp_accumulate(const char *q0, const char *q1, const char *q2, float frac,
+                                                  QuadWeights qw, float wc, float ws, float (&acc_re)[FPL],
+                                                  float (&acc_im)[FPL]) {
+    Taps<ST, INTERP, FPL> t;
+    load_taps<ST, INTERP, FPL>(q0, q1, q2, t);
+    combine_taps<ST, INTERP, FPL>(t, frac, qw, wc, ws, acc_re, acc_im);
 }
 
 // interp_accumulate at element offset `off` of `data`, taps `step` elements apart.
@@ -398,6 +484,10 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 // pair and broadcasts them, instead of every lane of the voxel forming them per element
                 // (the same float operations, so the same values).
                 constexpr bool kPreWeights = INTERP == kQuadratic && TC == 1;
+                // Two-element pipelining (below) for single-transmit launches whose taps of one element
+                // fit in 24 VGPRs, so that two elements' taps in flight leave room for 4+ waves per SIMD.
+                constexpr int kTapDwords = Taps<ST, INTERP, FPL>::kN * CH * FPL * kElemBytes<ST> / 4;
+                constexpr bool kPipe = TC == 1 && !WIDE && kTapDwords <= 24;
                 QuadWeights pw_own{0.f, 0.f, 0.f};
                 if constexpr (kPreWeights) pw_own = quad_weights(s[0].frac);
                 // Only elements that some voxel of this wave uses under some transmit of the chunk:
@@ -415,27 +505,68 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 }
                 // Summation order per voxel and frame: (transmit chunk, element, transmit) -- the same
                 // for every layout, so all layouts (and the batched kernel) give bit-identical results.
-                while (jm != 0ull) {
-                    const int j = __builtin_ctzll(jm);
-                    jm &= jm - 1ull;
-                    const int64_t e_abs = e0 + j;
+                if constexpr (kPipe) {
+                    // Two active elements per trip: the broadcasts and tap loads of both are issued
+                    // before either is combined, so a wave keeps twice the gathers in flight. They are
+                    // still combined element by element, in the order above.
+                    constexpr int kE = kElemBytes<ST>;
+                    const uint32_t fo = (uint32_t)f0 * (uint32_t)kE;
+                    const char *base = (const char *)p.data + t0 * p.n_rx * p.n_samples * p.n_frames * kE;
+                    const int64_t row_bytes = p.n_samples * p.n_frames * kE, step = p.tap_step * kE;
+                    // Every lane loads (straight-line code keeps the compiler's wait counts independent of
+                    // which lanes contribute): a lane without a contribution reads the first frames of
+                    // the element's first samples, which exist whenever some pair of the element is
+                    // valid, and discards them.
+                    auto load_elem = [&](int j, const ElemCoef &cf, bool ok, Taps<ST, INTERP, FPL>(&t)[CH]) {
+                        const char *r0 = base + (e0 + j) * row_bytes;
+                        const uint32_t o = (uint32_t)cf.i0 * (uint32_t)p.n_frames * (uint32_t)kE + fo;
 #pragma unroll
-                    for (int tt = 0; tt < TC; ++tt) {
-                        if (tt >= nt) break;
-                        const int i0 = bcast<LF>(s[tt].i0, j, lane);
-                        float frac = 0.f;
-                        QuadWeights qw{0.f, 0.f, 0.f};
-                        if constexpr (kPreWeights) {
-                            qw.l0 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l0), j, lane));
-                            qw.l1 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l1), j, lane));
-                            qw.l2 = __int_as_float(bcast<LF>(__float_as_int(pw_own.l2), j, lane));
-                        } else if constexpr (INTERP != kNearest) {
-                            frac = __int_as_float(bcast<LF>(__float_as_int(s[tt].frac), j, lane));
-                            if constexpr (INTERP == kQuadratic) qw = quad_weights(frac);
+                        for (int c = 0; c < CH; ++c) {
+                            uint32_t oc = (ok && (c == 0 || fok[c])) ? o + (uint32_t)(c * kGroup * kE) : 0u;
+                            asm volatile("" : "+v"(oc));  // keeps the select on the 32-bit lane offset
+                            load_taps<ST, INTERP, FPL>(r0 + oc, r0 + step + oc, r0 + 2 * step + oc, t[c]);
                         }
-                        const float wc = __int_as_float(bcast<LF>(__float_as_int(s[tt].wc), j, lane));
-                        const float ws = __int_as_float(bcast<LF>(__float_as_int(s[tt].ws), j, lane));
-                        if (i0 >= 0 && fok[0]) {
+                    };
+                    auto combine_elem = [&](const ElemCoef &cf, const Taps<ST, INTERP, FPL>(&t)[CH]) {
+#pragma unroll
+                        for (int c = 0; c < CH; ++c)
+                            if (c == 0 || fok[c])
+                                combine_taps<ST, INTERP, FPL>(t[c], cf.frac, cf.qw, cf.wc, cf.ws, acc_re[c], acc_im[c]);
+                    };
+                    while (jm != 0ull) {
+                        const int ja = __builtin_ctzll(jm);
+                        jm &= jm - 1ull;
+                        const bool two = jm != 0ull;  // wave-uniform; without a second element B repeats A
+                        const int jb = two ? __builtin_ctzll(jm) : ja;
+                        jm &= jm - 1ull;  // no-op when jm == 0
+                        const ElemCoef ca = elem_coef<LF, INTERP, kPreWeights>(s[0], pw_own, ja, lane);
+                        const ElemCoef cb = elem_coef<LF, INTERP, kPreWeights>(s[0], pw_own, jb, lane);
+                        const bool oka = ca.i0 >= 0 && fok[0];
+                        const bool okb = two && cb.i0 >= 0 && fok[0];
+                        Taps<ST, INTERP, FPL> ta[CH], tb[CH];
+                        load_elem(ja, ca, oka, ta);
+                        load_elem(jb, cb, okb, tb);
+                        if (oka) combine_elem(ca, ta);
+                        if (okb) combine_elem(cb, tb);
+                        // The taps count as used at the end of the trip, so the compiler can neither sink
+                        // a load into its combine (which would serialise the two elements again) nor carry
+                        // an unfinished load into the next trip.
+#pragma unroll
+                        for (int c = 0; c < CH; ++c) {
+                            keep_taps(ta[c]);
+                            keep_taps(tb[c]);
+                        }
+                    }
+                } else {
+                    while (jm != 0ull) {
+                        const int j = __builtin_ctzll(jm);
+                        jm &= jm - 1ull;
+                        const int64_t e_abs = e0 + j;
+#pragma unroll
+                        for (int tt = 0; tt < TC; ++tt) {
+                            if (tt >= nt) break;
+                            const ElemCoef cf = elem_coef<LF, INTERP, kPreWeights>(s[tt], pw_own, j, lane);
+                            if (cf.i0 < 0 || !fok[0]) continue;
                             // first element of the channel's [sample][frame] block (wave-uniform)
                             const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;
                             if constexpr (!WIDE) {
@@ -443,24 +574,25 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                                 const char *r0 = (const char *)p.data + row0 * kE;
                                 const char *r1 = r0 + p.tap_step * kE;
                                 const char *r2 = r1 + p.tap_step * kE;
-                                const uint32_t o = ((uint32_t)i0 * (uint32_t)p.n_frames + (uint32_t)f0) * (uint32_t)kE;
+                                const uint32_t o =
+                                    ((uint32_t)cf.i0 * (uint32_t)p.n_frames + (uint32_t)f0) * (uint32_t)kE;
 #pragma unroll
                                 for (int c = 0; c < CH; ++c) {
                                     const uint32_t oc = o + (uint32_t)(c * kGroup * kE);
                                     if (c == 0 || fok[c])
-                                        interp_accumulate<ST, INTERP, FPL>(r0 + oc, r1 + oc, r2 + oc, frac, qw, wc,
-                                                                           ws, acc_re[c], acc_im[c]);
+                                        interp_accumulate<ST, INTERP, FPL>(r0 + oc, r1 + oc, r2 + oc, cf.frac, cf.qw,
+                                                                           cf.wc, cf.ws, acc_re[c], acc_im[c]);
                                 }
                             } else {
-                                const int64_t off = row0 + f0 + (int64_t)i0 * p.n_frames;
+                                const int64_t off = row0 + f0 + (int64_t)cf.i0 * p.n_frames;
+                                const int64_t sb = p.tap_step * kElemBytes<ST>;
 #pragma unroll
                                 for (int c = 0; c < CH; ++c) {
                                     if (c == 0 || fok[c]) {
-                                        const char *q0 = (const char *)p.data + (off + (int64_t)c * kGroup) *
-                                                                                    kElemBytes<ST>;
-                                        const int64_t sb = p.tap_step * kElemBytes<ST>;
-                                        interp_accumulate<ST, INTERP, FPL>(q0, q0 + sb, q0 + 2 * sb, frac, qw, wc, ws,
-                                                                           acc_re[c], acc_im[c]);
+                                        const char *q0 =
+                                            (const char *)p.data + (off + (int64_t)c * kGroup) * kElemBytes<ST>;
+                                        interp_accumulate<ST, INTERP, FPL>(q0, q0 + sb, q0 + 2 * sb, cf.frac, cf.qw,
+                                                                           cf.wc, cf.ws, acc_re[c], acc_im[c]);
                                     }
                                 }
                             }
