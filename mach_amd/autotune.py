@@ -30,9 +30,19 @@ DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning
 MULTI_GROUP = {(16, 4), (32, 2), (64, 1), (8, 4)}
 
 
+# lateral groups: 64 lanes, every lane holding its frames of 2 or 4 voxels that share sample windows
+LATERAL_FLAG = 0x8000
+LATERAL = ((2, 2), (2, 4), (4, 2), (4, 4))  # (voxels_per_lane, frames_per_lane)
+
+
 def encode(lanes_per_voxel: int, frames_per_lane: int, chunks_per_lane: int = 1) -> int:
     """C-ABI layout code: (log2(chunks_per_lane) << 12) | (lanes_per_voxel << 4) | frames_per_lane."""
     return ((chunks_per_lane.bit_length() - 1) << 12) | (lanes_per_voxel << 4) | frames_per_lane
+
+
+def encode_lateral(voxels_per_lane: int, frames_per_lane: int) -> int:
+    """C-ABI code of a lateral-group layout: 0x8000 | (log2(voxels_per_lane) << 12) | (64 << 4) | frames."""
+    return LATERAL_FLAG | ((voxels_per_lane.bit_length() - 1) << 12) | (64 << 4) | frames_per_lane
 
 
 def decode(layout: int):
@@ -41,7 +51,11 @@ def decode(layout: int):
 
 
 def chunks_per_lane(layout: int) -> int:
-    return 1 << ((layout >> 12) & 0xF)
+    return 1 if layout & LATERAL_FLAG else 1 << ((layout >> 12) & 0xF)
+
+
+def voxels_per_lane(layout: int) -> int:
+    return 1 << ((layout >> 12) & 0x7) if layout & LATERAL_FLAG else 1
 
 
 def _bucket(n: int) -> int:
@@ -55,12 +69,16 @@ def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox
     return key + (f":o{int(output)}" if output else "")
 
 
-def candidates(n_frames: int, band: bool = True):
+def candidates(n_frames: int, band: bool = True, lateral: bool = False):
     """Layouts worth timing for ``n_frames``: frames-per-lane divides the frame count and a voxel's
     lanes cover no more than twice the frames they have to share (or 8 lanes: extra lanes still split
     the per-pair geometry of setup-bound, few-frame problems). With ``band``, the LDS-staged band
-    kernel (lanes_per_voxel code 0) is included where its 64-frame rows divide the frame count."""
+    kernel (lanes_per_voxel code 0) is included where its 64-frame rows divide the frame count; with
+    ``lateral`` (single-transmit requests), the lateral-group layouts whose 64 lanes are at least half
+    busy."""
     out = []
+    if lateral:
+        out += [encode_lateral(vpl, k) for vpl, k in LATERAL if n_frames % k == 0 and 32 * k <= n_frames]
     if band:
         out += [encode(0, k) for k in FRAMES_PER_LANE if n_frames % (64 * k) == 0]
     for k in FRAMES_PER_LANE:
@@ -90,10 +108,10 @@ def time_layout(run, layout: int, reps: int = 5, warmup: int = 1) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(run, n_frames: int, reps: int = 5, band: bool = True):
+def tune(run, n_frames: int, reps: int = 5, band: bool = True, lateral: bool = False):
     """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms})."""
     times = {}
-    for lay in candidates(n_frames, band=band):
+    for lay in candidates(n_frames, band=band, lateral=lateral):
         try:
             times[lay] = time_layout(run, lay, reps)
         except RuntimeError:  # layout rejected for this request (alignment)
@@ -190,7 +208,7 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = Tr
 
     # the band kernel sums transmits in a different order and has no epilogues: keep it to
     # single-transmit signal requests, so that a tuned layout never changes results
-    best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1 and output == 0)
+    best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1 and output == 0, lateral=acq.n_tx == 1)
     key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox, output)
     if record:
         table().record(key, best, times, {"workload": acq.name, "storage": storage,

@@ -270,26 +270,13 @@ __device__ __forceinline__ void load_taps(const char *q0, const char *q1, const 
     }
 }
 
-// Interpolates loaded taps and adds w * exp(j phase) * value to the accumulators. `frac` is the lerp
-// fraction (linear); quadratic interpolation takes its weights `qw`.
-template <int ST, int INTERP, int FPL>
-__device__ __forceinline__ void combine_taps(const Taps<ST, INTERP, FPL> &t, float frac, QuadWeights qw, float wc,
-                                             float ws, float (&acc_re)[FPL], float (&acc_im)[FPL]) {
-    constexpr bool kComplex = ST != kStoreF32;
-    constexpr int kN = Taps<ST, INTERP, FPL>::kN;
-    float r[kN][FPL], i[kN][FPL];
-#pragma unroll
-    for (int n = 0; n < kN; ++n)
-#pragma unroll
-        for (int k = 0; k < FPL; ++k) {
-            if constexpr (ST == kStoreC32H) {
-                r[n][k] = (float)t.h[n][2 * k];
-                i[n][k] = (float)t.h[n][2 * k + 1];
-            } else {
-                r[n][k] = t.re[n][k];
-                i[n][k] = t.im[n][k];
-            }
-        }
+// Interpolates kN float rows (row n = sample i0 + n; re/im per frame) and adds w * exp(j phase) * value
+// to the accumulators. `frac` is the lerp fraction (linear); quadratic interpolation takes its weights
+// `qw`. Rows are passed as row pointers so a caller can pass a sub-window of a larger set of rows.
+template <bool kComplex, int INTERP, int FPL>
+__device__ __forceinline__ void combine_rows(const float *const (&r)[3], const float *const (&i)[3], float frac,
+                                             QuadWeights qw, float wc, float ws, float (&acc_re)[FPL],
+                                             float (&acc_im)[FPL]) {
     float vr[FPL], vi[FPL];
     if constexpr (INTERP == kNearest) {
 #pragma unroll
@@ -319,6 +306,28 @@ __device__ __forceinline__ void combine_taps(const Taps<ST, INTERP, FPL> &t, flo
             acc_re[k] = fmaf(wc, vr[k], acc_re[k]);
         }
     }
+}
+
+// Interpolates loaded taps and adds w * exp(j phase) * value to the accumulators (combine_rows).
+template <int ST, int INTERP, int FPL>
+__device__ __forceinline__ void combine_taps(const Taps<ST, INTERP, FPL> &t, float frac, QuadWeights qw, float wc,
+                                             float ws, float (&acc_re)[FPL], float (&acc_im)[FPL]) {
+    constexpr int kN = Taps<ST, INTERP, FPL>::kN;
+    float r[3][FPL], i[3][FPL];
+#pragma unroll
+    for (int n = 0; n < kN; ++n)
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            if constexpr (ST == kStoreC32H) {
+                r[n][k] = (float)t.h[n][2 * k];
+                i[n][k] = (float)t.h[n][2 * k + 1];
+            } else {
+                r[n][k] = t.re[n][k];
+                i[n][k] = t.im[n][k];
+            }
+        }
+    const float *const rp[3] = {r[0], r[1], r[2]}, *const ip[3] = {i[0], i[1], i[2]};
+    combine_rows<ST != kStoreF32, INTERP, FPL>(rp, ip, frac, qw, wc, ws, acc_re, acc_im);
 }
 
 // An empty use of loaded taps (no instructions): it pins the loads ahead of this point.
@@ -392,6 +401,59 @@ __device__ __forceinline__ float wave_max(float x) {
 // unsigned atomic max gives the maximum whatever the order of the updates.
 __device__ __forceinline__ void peak_max(float *peak, float x) {
     atomicMax((unsigned int *)peak, __float_as_uint(x));
+}
+
+// Epilogue of one voxel's FPL frames from frame fc on (p.output): stores the signal (out (+)= y) or
+// the envelope |y|, or adds sum |y|^2 to `pw` for the power image. Raises vmax to the envelope values.
+template <int ST, int FPL>
+__device__ __forceinline__ void emit_frames(const DasParams &p, int64_t v, int64_t fc, const float (&acc_re)[FPL],
+                                            const float (&acc_im)[FPL], float &pw, float &vmax) {
+    constexpr bool kComplex = ST != kStoreF32;
+    if (p.output == kOutPower) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            pw = fmaf(acc_re[k], acc_re[k], pw);
+            if constexpr (kComplex) pw = fmaf(acc_im[k], acc_im[k], pw);
+        }
+    } else if (p.output == kOutEnvelope) {
+        float *q = (float *)p.out + v * p.n_frames + fc;
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            const float a = kComplex ? sqrtf(fmaf(acc_re[k], acc_re[k], acc_im[k] * acc_im[k])) : fabsf(acc_re[k]);
+            q[k] = a;
+            vmax = fmaxf(vmax, a);
+        }
+    } else {
+        const int64_t o = v * p.n_frames + fc;
+        if constexpr (ST == kStoreF32) {
+            float *q = (float *)p.out + o;
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) q[k] = p.accumulate ? q[k] + acc_re[k] : acc_re[k];
+        } else {
+            float *q = (float *)p.out + 2 * o;
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) {
+                q[2 * k + 0] = p.accumulate ? q[2 * k + 0] + acc_re[k] : acc_re[k];
+                q[2 * k + 1] = p.accumulate ? q[2 * k + 1] + acc_im[k] : acc_im[k];
+            }
+        }
+    }
+}
+
+// Sum of x over each aligned group of LF lanes (xor butterfly; every lane of the group gets it).
+template <int LF>
+__device__ __forceinline__ float lane_group_sum(float x) {
+#pragma unroll
+    for (int m = 1; m < LF; m <<= 1) x += __shfl_xor(x, m);
+    return x;
+}
+
+// Writes voxel v's power value (out[v] (+)= pw) and returns it.
+__device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float pw) {
+    float *q = (float *)p.out + v;
+    const float r = p.accumulate ? *q + pw : pw;
+    *q = r;
+    return r;
 }
 
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
@@ -603,54 +665,13 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
         }
 
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const int64_t fc = f0 + (int64_t)c * kGroup;
-            if (power) {
-                if (vok && fok[c]) {
-#pragma unroll
-                    for (int k = 0; k < FPL; ++k) {
-                        pw = fmaf(acc_re[c][k], acc_re[c][k], pw);
-                        if constexpr (kComplex) pw = fmaf(acc_im[c][k], acc_im[c][k], pw);
-                    }
-                }
-            } else if (p.output == kOutEnvelope) {
-                if (vok && fok[c]) {
-                    float *q = (float *)p.out + v * p.n_frames + fc;
-#pragma unroll
-                    for (int k = 0; k < FPL; ++k) {
-                        const float a = kComplex ? sqrtf(fmaf(acc_re[c][k], acc_re[c][k], acc_im[c][k] * acc_im[c][k]))
-                                                 : fabsf(acc_re[c][k]);
-                        q[k] = a;
-                        vmax = fmaxf(vmax, a);
-                    }
-                }
-            } else if (vok && fok[c]) {
-                const int64_t o = v * p.n_frames + fc;
-                if constexpr (ST == kStoreF32) {
-                    float *q = (float *)p.out + o;
-#pragma unroll
-                    for (int k = 0; k < FPL; ++k) q[k] = p.accumulate ? q[k] + acc_re[c][k] : acc_re[c][k];
-                } else {
-                    float *q = (float *)p.out + 2 * o;
-#pragma unroll
-                    for (int k = 0; k < FPL; ++k) {
-                        q[2 * k + 0] = p.accumulate ? q[2 * k + 0] + acc_re[c][k] : acc_re[c][k];
-                        q[2 * k + 1] = p.accumulate ? q[2 * k + 1] + acc_im[c][k] : acc_im[c][k];
-                    }
-                }
-            }
-        }
+        for (int c = 0; c < CH; ++c)
+            if (vok && fok[c]) emit_frames<ST, FPL>(p, v, f0 + (int64_t)c * kGroup, acc_re[c], acc_im[c], pw, vmax);
     }
 
     if (power) {
-#pragma unroll
-        for (int m = 1; m < LF; m <<= 1) pw += __shfl_xor(pw, m);  // lanes lv*LF .. lv*LF+LF-1
-        if (vok && lf == 0) {
-            float *q = (float *)p.out + v;
-            const float r = p.accumulate ? *q + pw : pw;
-            *q = r;
-            vmax = r;
-        }
+        pw = lane_group_sum<LF>(pw);  // lanes lv*LF .. lv*LF+LF-1
+        if (vok && lf == 0) vmax = emit_power(p, v, pw);
     }
     if (p.output != kOutSignal && p.peak != nullptr) {
         const float m = wave_max(vmax);

@@ -59,11 +59,14 @@ struct DasParams {
 };
 
 // Frame-lane layout of a launch: LF lanes of a wave share one voxel and split its frames, each lane
-// handling FPL consecutive frames; a wave therefore covers 64 / LF voxels.
+// handling FPL consecutive frames; a wave therefore covers 64 / LF voxels. With voxels_per_lane
+// VPL > 1 (lateral groups, LF = 64) every lane instead holds its FPL frames of VPL voxels that are
+// consecutive in the processing order, and the wave reads their taps from shared sample windows.
 struct DasLayout {
     int lanes_per_voxel;      // LF in {1, 2, 4, 8, 16, 32, 64}
     int frames_per_lane;      // FPL in {1, 2, 4}
     int chunks_per_lane = 1;  // CH in {1, 2, 4}: frame groups of LF * FPL frames a lane walks per setup
+    int voxels_per_lane = 1;  // VPL in {1, 2, 4}; VPL > 1 only with LF = 64, FPL in {2, 4}, CH = 1
 };
 
 // (LF, FPL) layouts compiled with CH > 1 (frame ensembles, where the per-pair setup is worth sharing).
@@ -72,7 +75,12 @@ constexpr bool multi_group_layout(int lf, int fpl) {
 }
 
 // Layout code of the C ABI: (log2(CH) << 12) | (LF << 4) | FPL; LF == 0 selects the band kernel.
+// Lateral groups: 0x8000 | (log2(VPL) << 12) | (64 << 4) | FPL.
 inline DasLayout decode_layout(int32_t code) {
+    if (code & 0x8000) {
+        const int vpl_log2 = (code >> 12) & 0x7;
+        return DasLayout{(code >> 4) & 0xff, code & 0xf, 1, vpl_log2 <= 2 ? 1 << vpl_log2 : 0};
+    }
     const int ch_log2 = (code >> 12) & 0xf;
     return DasLayout{(code >> 4) & 0xff, code & 0xf, ch_log2 <= 2 ? 1 << ch_log2 : 0};
 }

@@ -25,6 +25,11 @@ uintptr_t out_align(const DasParams &p, int store, int k) {
 
 bool layout_valid(const DasParams &p, int store, DasLayout l) {
     const int lf = l.lanes_per_voxel, k = l.frames_per_lane, ch = l.chunks_per_lane;
+    if (l.voxels_per_lane != 1) {  // lateral groups: single transmit, 64 lanes, 2 or 4 frames per lane
+        if ((l.voxels_per_lane != 2 && l.voxels_per_lane != 4) || lf != 64 || (k != 2 && k != 4) || ch != 1 ||
+            p.n_tx != 1)
+            return false;
+    }
     if (lf < 1 || lf > 64 || (lf & (lf - 1)) != 0) return false;
     if (k != 1 && k != 2 && k != 4) return false;
     if (ch != 1 && !((ch == 2 || ch == 4) && multi_group_layout(lf, k))) return false;

@@ -1,0 +1,174 @@
+// Lateral-group delay-and-sum block: one wave beamforms VPL voxels that are consecutive in the
+// processing order (with the depth-major order: lateral neighbours at one depth), for 64 * FPL frames
+// (lane l holds frames l * FPL .. l * FPL + FPL - 1 of the block for every voxel of the group).
+//
+// Why: the general kernel's gathers are bound by the vector L1's delivery rate (one 64-byte line per
+// clock per CU): every voxel reads its own kN interpolation rows per element. Neighbouring voxels at
+// one depth have nearly the same delay to an element (their first taps differ by at most a sample or
+// two), so the group's taps fall in one window of kN + kSpread rows. The wave loads that window once
+// and each voxel interpolates from its own rows of it: for quadratic interpolation 4 rows serve 2
+// voxels and 5 rows serve 4, where the general kernel reads 6 and 12. Elements whose tap starts spread
+// wider than the window are read voxel by voxel.
+//
+// Everything per voxel is wave-uniform here (each voxel's setup is broadcast with v_readlane into
+// scalar registers), so the window position and each voxel's offset in it are scalar branches.
+// Each voxel and frame is summed element by element with the general kernel's float operations
+// (combine_rows), so the result is bit-identical to every other layout. Single-transmit launches only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "das_device.h"
+
+namespace mach {
+namespace dev {
+
+// Combines window rows K .. K + 2 (clamped to the window; only the first kN are read) into one voxel.
+template <bool kComplex, int INTERP, int FPL, int K, int kW>
+__device__ __forceinline__ void combine_window(const float (&wr)[kW][FPL], const float (&wi)[kW][FPL],
+                                               const ElemCoef &cf, float (&acc_re)[FPL], float (&acc_im)[FPL]) {
+    constexpr int a = K < kW ? K : kW - 1, b = K + 1 < kW ? K + 1 : kW - 1, c = K + 2 < kW ? K + 2 : kW - 1;
+    const float *const rp[3] = {wr[a], wr[b], wr[c]}, *const ip[3] = {wi[a], wi[b], wi[c]};
+    combine_rows<kComplex, INTERP, FPL>(rp, ip, cf.frac, cf.qw, cf.wc, cf.ws, acc_re, acc_im);
+}
+
+template <int ST, int INTERP, int VPL, int FPL>
+__device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t bx, uint32_t by) {
+    constexpr int kN = INTERP == kNearest ? 1 : (INTERP == kLinear ? 2 : 3);  // rows per voxel
+    constexpr int kSpread = VPL == 2 ? 1 : 2;  // widest spread of tap starts served by one window
+    constexpr int kW = kN + kSpread;           // rows per window
+    constexpr int kEC = 64 / VPL;              // elements per setup round (setup lanes per voxel)
+    constexpr int kE = kElemBytes<ST>;
+    constexpr int kGroup = 64 * FPL;  // frames per block
+    constexpr bool kComplex = ST != kStoreF32;
+    constexpr bool kPre = INTERP == kQuadratic;  // the setup lane forms the Lagrange weights
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t first = ((int64_t)bx * 4 + wave) * VPL;  // first slot of the wave's group
+
+    // The group's voxels (wave-uniform), for the epilogue.
+    int64_t vq[VPL];
+    bool okq[VPL];
+#pragma unroll
+    for (int q = 0; q < VPL; ++q) {
+        const int64_t slot = first + q;
+        okq[q] = slot < p.n_vox;
+        vq[q] = okq[q] ? (p.order != nullptr ? (int64_t)p.order[slot] : slot) : 0;
+    }
+
+    // Setup role of this lane: voxel q_s of the group, element slot j_s of each round of kEC elements.
+    const int q_s = lane / kEC, j_s = lane % kEC;
+    const int64_t slot_s = first + q_s;
+    const bool vok = slot_s < p.n_vox;
+    const int64_t v = (vok && p.order != nullptr) ? (int64_t)p.order[slot_s] : slot_s;
+    double xv = 0.0, yv = 0.0, zv = 1.0, Tv = 0.0;
+    if (vok) {
+        xv = (double)p.vox[3 * v + 0];
+        yv = (double)p.vox[3 * v + 1];
+        zv = (double)p.vox[3 * v + 2];
+        Tv = p.waves != nullptr ? wave_arrival(p.waves[0], xv, yv, zv) : (double)p.arrivals[v];
+    }
+    const double R = zv / p.two_f_number;  // aperture radius (kernel.cu:583)
+    const double R2 = R * R;
+    const double inv_R = 1.0 / R;
+
+    const int64_t row_bytes = p.n_samples * p.n_frames * kE, step = p.tap_step * kE;
+    const bool power = p.output == kOutPower;
+    const int64_t n_blocks_f = power ? (p.n_frames + kGroup - 1) / kGroup : 1;
+    float pw[VPL], vmax = 0.f;
+#pragma unroll
+    for (int q = 0; q < VPL; ++q) pw[q] = 0.f;
+    for (int64_t fb = 0; fb < n_blocks_f; ++fb) {
+        const int64_t f0 = (power ? fb : (int64_t)by) * kGroup + (int64_t)lane * FPL;
+        const bool fok = f0 < p.n_frames;
+        // lanes past the last frame read the first frames of the same rows and discard them
+        const uint32_t fo = fok ? (uint32_t)f0 * (uint32_t)kE : 0u;
+        float acc_re[VPL][FPL], acc_im[VPL][FPL];
+#pragma unroll
+        for (int q = 0; q < VPL; ++q)
+#pragma unroll
+            for (int k = 0; k < FPL; ++k) acc_re[q][k] = acc_im[q][k] = 0.f;
+
+        for (int64_t e0 = 0; e0 < p.n_rx; e0 += kEC) {
+            const int64_t e = e0 + j_s;
+            PairGeom g{0.0, 0.f};
+            if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+            const PairSetup s = pair_time<INTERP>(p, g, Tv);
+            QuadWeights pw_own{0.f, 0.f, 0.f};
+            if constexpr (kPre) pw_own = quad_weights(s.frac);
+            // element slots some voxel of the group uses (bit q * kEC + j: voxel q, element e0 + j)
+            uint64_t jm = __ballot(s.i0 >= 0);
+#pragma unroll
+            for (int sh = kEC; sh < 64; sh <<= 1) jm |= jm >> sh;
+            jm &= (1ull << kEC) - 1ull;
+            while (jm != 0ull) {
+                const int j = __builtin_ctzll(jm);
+                jm &= jm - 1ull;
+                const char *r0 = (const char *)p.data + (e0 + j) * row_bytes;
+                int i0[VPL];
+                int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+                for (int q = 0; q < VPL; ++q) {
+                    i0[q] = __builtin_amdgcn_readlane(s.i0, q * kEC + j);
+                    if (i0[q] >= 0) {
+                        lo = i0[q] < lo ? i0[q] : lo;
+                        hi = i0[q] > hi ? i0[q] : hi;
+                    }
+                }
+                if (hi - lo <= kSpread && p.n_samples >= kW) {
+                    // one window of kW rows, placed to stay inside the record
+                    const int start = lo < (int)p.n_samples - kW ? lo : (int)p.n_samples - kW;
+                    const char *rs = r0 + (int64_t)start * step;
+                    float wr[kW][FPL], wi[kW][FPL];
+#pragma unroll
+                    for (int r = 0; r < kW; ++r) load_at<ST, FPL>(rs + r * step + fo, wr[r], wi[r]);
+#pragma unroll
+                    for (int q = 0; q < VPL; ++q) {
+                        if (i0[q] < 0) continue;
+                        const ElemCoef cf = elem_coef<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane);
+                        const int k = i0[q] - start;
+                        if (k == 0) {
+                            combine_window<kComplex, INTERP, FPL, 0, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                        } else if (kSpread == 1 || k == 1) {
+                            combine_window<kComplex, INTERP, FPL, 1, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                        } else {
+                            combine_window<kComplex, INTERP, FPL, 2, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < VPL; ++q) {
+                        if (i0[q] < 0) continue;
+                        const ElemCoef cf = elem_coef<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane);
+                        const char *rq = r0 + (int64_t)i0[q] * step;
+                        float tr[kN][FPL], ti[kN][FPL];
+#pragma unroll
+                        for (int n = 0; n < kN; ++n) load_at<ST, FPL>(rq + n * step + fo, tr[n], ti[n]);
+                        combine_window<kComplex, INTERP, FPL, 0, kN>(tr, ti, cf, acc_re[q], acc_im[q]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < VPL; ++q)
+            if (okq[q] && fok) emit_frames<ST, FPL>(p, vq[q], f0, acc_re[q], acc_im[q], pw[q], vmax);
+    }
+
+    if (power) {
+#pragma unroll
+        for (int q = 0; q < VPL; ++q) {
+            const float t = lane_group_sum<64>(pw[q]);
+            if (okq[q] && lane == 0) vmax = fmaxf(vmax, emit_power(p, vq[q], t));
+        }
+    }
+    if (p.output != kOutSignal && p.peak != nullptr) {
+        const float m = wave_max(vmax);
+        if (lane == 0 && m > 0.f) peak_max(p.peak, m);
+    }
+}
+
+}  // namespace dev
+}  // namespace mach

@@ -171,7 +171,8 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
                                                    " is not valid for this request (lanes_per_voxel must be a power of"
                                                    " two <= 64, frames_per_lane in {1, 2, 4} dividing n_frames and"
                                                    " keeping vector accesses aligned, chunks_per_lane in {2, 4} only"
-                                                   " for 16x4, 32x2, 64x1 and 8x4)");
+                                                   " for 16x4, 32x2, 64x1 and 8x4; lateral groups of 2 or 4 voxels"
+                                                   " only for 64x2 and 64x4 and a single transmit)");
     out->layout = l;
     return MACH_OK;
 }

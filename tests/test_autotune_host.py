@@ -22,6 +22,22 @@ def test_candidates_divide_frames_and_encode():
     assert 0x2 in autotune.candidates(256) and 0x2 not in autotune.candidates(8)  # band kernel
 
 
+def test_lateral_layout_codes():
+    """Lateral groups: 0x8000 | log2(voxels per lane) << 12 | 64 << 4 | frames per lane, offered only
+    for single-transmit requests whose frames keep at least half of the 64 lanes busy."""
+    assert autotune.encode_lateral(2, 4) == 0x9404 and autotune.encode_lateral(4, 2) == 0xA402
+    for lay in (0x9404, 0xA402):
+        assert autotune.decode(lay)[0] == 64 and autotune.chunks_per_lane(lay) == 1
+    assert autotune.voxels_per_lane(0x9404) == 2 and autotune.voxels_per_lane(0xA404) == 4
+    assert autotune.voxels_per_lane(0x2104) == 1
+    lat256 = [lay for lay in autotune.candidates(256, lateral=True) if lay & autotune.LATERAL_FLAG]
+    assert sorted(lat256) == [0x9402, 0x9404, 0xA402, 0xA404]
+    assert not any(lay & autotune.LATERAL_FLAG for lay in autotune.candidates(256))
+    lat64 = [lay for lay in autotune.candidates(64, lateral=True) if lay & autotune.LATERAL_FLAG]
+    assert sorted(lat64) == [0x9402, 0xA402]  # 64 x 4 frames would leave 48 of 64 lanes idle
+    assert not any(lay & autotune.LATERAL_FLAG for lay in autotune.candidates(32, lateral=True))
+
+
 def test_shape_key_buckets_by_octave():
     a = autotune.shape_key(1, 2, 1, 128, 256, 65536)
     b = autotune.shape_key(1, 2, 1, 100, 256, 40000)

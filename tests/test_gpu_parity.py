@@ -331,3 +331,86 @@ def test_band_kernel_matches_plain_and_oracle(device, interp, kind, frames):
                     modulation_freq_hz=fmod, tukey_alpha=0.5, interp_type=interp)
         assert np.abs(ref).max() > 0
         assert rel_l2(band.cpu().numpy()[idx], ref) <= TOL_FP32
+
+
+def _beamform_raw(cd, rx, scan, arr, out, args, layout, output=0, order=None):
+    """mach_beamform with a forced layout and output kind (0 signal, 1 power, 2 envelope)."""
+    from mach_amd import _hip_impl
+
+    kind = _hip_impl._data_kind(cd)
+    a = _hip_impl.make_args(cd, rx, scan, arr, out, n_tx=arr.shape[0], kind=kind, f_number=args[0],
+                            rx_start_s=args[1], sampling_freq_hz=args[2], sound_speed_m_s=args[3],
+                            modulation_freq_hz=args[4], tukey_alpha=args[5], interp_type=args[6], accumulate=False,
+                            layout=layout, order=order, output=output)
+    _lib.check(_lib.load().mach_beamform(ctypes.byref(a), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2])
+@pytest.mark.parametrize("kind", ["iq", "iq16", "rf"])
+@pytest.mark.parametrize("frames", [64, 256])
+def test_lateral_layouts_match_general_and_oracle(device, interp, kind, frames):
+    """Lateral-group layouts (2 or 4 voxels per lane reading shared sample windows) against the general
+    kernel: bitwise equal for the signal and envelope outputs and, against the 64-lane general layout
+    with the same frames per lane (same per-lane sums and butterfly), for the power image; and against
+    the oracle on a voxel sample. The regular grid in depth-major order takes the window path; a
+    random cloud (ordered and in caller order) also exercises the voxel-by-voxel fallback."""
+    from mach_amd import _hip_impl, autotune
+
+    log(f"lateral interp={interp} kind={kind} frames={frames}")
+    acq = phantoms.doppler(device=device, scale=0.25, n_frames=frames)
+    rng = np.random.default_rng(frames + 7 * interp)
+    cloud = torch.as_tensor((rng.normal(size=(4100, 3)) * [8e-3, 0, 10e-3] + [0, 0, 25e-3]).astype(np.float32),
+                            device=device)
+    cloud_arr = (cloud[:, 2] / 1540.0)[None].contiguous()
+    cd = acq.channel_data
+    if kind == "iq16":
+        cd = acq.fp16_channel_data()
+    elif kind == "rf":
+        cd = acq.channel_data.real.contiguous()
+    otype = torch.float32 if kind == "rf" else torch.complex64
+    fmod = 0.0 if kind == "rf" else acq.params["modulation_freq_hz"]
+    args = (acq.params["f_number"], 0.0, acq.params["sampling_freq_hz"], acq.params["sound_speed_m_s"], fmod, 0.5,
+            interp)
+    lateral = [lay for lay in autotune.candidates(frames, band=False, lateral=True) if lay & autotune.LATERAL_FLAG]
+    assert lateral
+    cases = ((acq.scan_coords_m, acq.tx_wave_arrivals_s, True), (cloud, cloud_arr, True), (cloud, cloud_arr, False))
+    for scan, arr, ordered in cases:
+        order = _hip_impl.voxel_order(scan) if ordered else None
+        n = scan.shape[0]
+        ref_sig = torch.zeros((n, frames), dtype=otype, device=device)
+        _beamform_raw(cd, acq.rx_coords_m, scan, arr, ref_sig, args, autotune.encode(32, 2), order=order)
+        ref_env = torch.zeros((n, frames), dtype=torch.float32, device=device)
+        _beamform_raw(cd, acq.rx_coords_m, scan, arr, ref_env, args, autotune.encode(32, 2), 2, order)
+        for lay in lateral:
+            k = lay & 0xF
+            sig = torch.zeros_like(ref_sig)
+            _beamform_raw(cd, acq.rx_coords_m, scan, arr, sig, args, lay, order=order)
+            env = torch.zeros_like(ref_env)
+            _beamform_raw(cd, acq.rx_coords_m, scan, arr, env, args, lay, 2, order)
+            pw = torch.zeros((n,), dtype=torch.float32, device=device)
+            _beamform_raw(cd, acq.rx_coords_m, scan, arr, pw, args, lay, 1, order)
+            pw_ref = torch.zeros_like(pw)
+            _beamform_raw(cd, acq.rx_coords_m, scan, arr, pw_ref, args, autotune.encode(64, k), 1, order)
+            torch.cuda.synchronize()
+            assert torch.equal(sig, ref_sig), hex(lay)
+            assert torch.equal(env, ref_env), hex(lay)
+            assert torch.equal(pw, pw_ref), hex(lay)
+        idx = np.sort(rng.choice(n, 300, replace=False))
+        ref = das_c(cd.cpu().numpy(), acq.rx_coords_m.cpu().numpy(), scan.cpu().numpy()[idx], arr.cpu().numpy()[:, idx],
+                    f_number=args[0], rx_start_s=0.0, sampling_freq_hz=args[2], sound_speed_m_s=args[3],
+                    modulation_freq_hz=fmod, tukey_alpha=0.5, interp_type=interp)
+        assert np.abs(ref).max() > 0
+        assert rel_l2(sig.cpu().numpy()[idx], ref) <= TOL_FP32
+
+
+def test_lateral_layout_rejected_for_several_transmits(device):
+    """Lateral groups sum a single transmit; with two transmits the layout is refused, not approximated."""
+    from mach_amd import _hip_impl, autotune
+
+    rng = np.random.default_rng(3)
+    D, rx, vox, arr = random_case(rng, T=2, E=9, S=48, F=64, V=150)
+    d, r, v, a = to_dev(D, rx, vox, arr, device=device)
+    out = torch.zeros((150, 64), dtype=torch.complex64, device=device)
+    with pytest.raises(RuntimeError, match="lateral"):
+        _hip_impl.beamform_compound(d, r, v, a, out, 0.8, -1e-7, 20e6, 1540.0, 5e6, 0.5, 2,
+                                    layout=autotune.encode_lateral(2, 2))
