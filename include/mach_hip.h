@@ -110,7 +110,12 @@ typedef struct mach_beamform_args {
                                          lanes_per_voxel 0 (0x001, 0x002, 0x004) selects the
                                          LDS-staged kernel for frame ensembles, which needs a
                                          voxel_order and n_frames a multiple of 64 x
-                                         frames_per_lane. Every layout gives bit-identical
+                                         frames_per_lane. 0x8000 | (log2(voxels_per_lane)
+                                         << 12) | 0x400 | frames_per_lane (0x9402, 0x9404,
+                                         0xA402, 0xA404) selects lateral groups: each lane
+                                         holds its frames of 2 or 4 voxels consecutive in
+                                         voxel_order, read from shared sample windows (single
+                                         transmit only). Every layout gives bit-identical
                                          results for single-transmit requests; general-kernel
                                          layouts also for several transmits.                   */
     const int32_t *voxel_order;       /* optional device [n_vox] permutation of 0..n_vox-1: the
