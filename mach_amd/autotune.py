@@ -94,7 +94,7 @@ def candidates(n_frames: int, band: bool = True, lateral: bool = False):
     return out
 
 
-def time_layout(run, layout: int, reps: int = 5, warmup: int = 1) -> float:
+def time_layout(run, layout: int, reps: int = 20, warmup: int = 3) -> float:
     """Average milliseconds of ``run(layout)`` (one launch) over ``reps``, HIP events on the stream."""
     for _ in range(warmup):
         run(layout)
@@ -108,7 +108,7 @@ def time_layout(run, layout: int, reps: int = 5, warmup: int = 1) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(run, n_frames: int, reps: int = 5, band: bool = True, lateral: bool = False):
+def tune(run, n_frames: int, reps: int = 20, band: bool = True, lateral: bool = False):
     """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms})."""
     times = {}
     for lay in candidates(n_frames, band=band, lateral=lateral):
@@ -178,7 +178,7 @@ def layout_for(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vo
     return lay
 
 
-def tune_acquisition(acq, storage: str = "c64", reps: int = 5, record: bool = True, output: int = 0):
+def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = True, output: int = 0):
     """Tunes the layout for a synthetic acquisition (``mach_amd.phantoms.Acquisition``) and one output
     kind (mach_output_kind: 0 signal, 1 power Doppler image, 2 envelope)."""
     import ctypes
@@ -224,7 +224,7 @@ def _main():
     ap = argparse.ArgumentParser(description="Tune kernel layouts for the benchmark workloads and save the table")
     ap.add_argument("--workloads", default="plane_wave,compound,doppler,volumetric,pymust_disk")
     ap.add_argument("--out", default=None)
-    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--power", default="doppler,pymust_disk", help="workloads also tuned for the power Doppler output")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)

@@ -291,10 +291,14 @@ __device__ __forceinline__ void combine_rows(const float *const (&r)[3], const f
             if constexpr (kComplex) vi[k] = fmaf(frac, i[1][k] - i[0][k], i[0][k]);
         }
     } else {
+        // The first product is written as fmaf(l0, a, 0), which rounds like l0 * a (a zero product
+        // may lose its sign, which can change nothing but the sign of an exactly zero output): with
+        // FP16 storage all three taps then feed mixed-precision FMAs (v_fma_mix_f32) straight from
+        // the packed halves, with no conversions.
 #pragma unroll
         for (int k = 0; k < FPL; ++k) {
-            vr[k] = fmaf(qw.l2, r[2][k], fmaf(qw.l1, r[1][k], qw.l0 * r[0][k]));
-            if constexpr (kComplex) vi[k] = fmaf(qw.l2, i[2][k], fmaf(qw.l1, i[1][k], qw.l0 * i[0][k]));
+            vr[k] = fmaf(qw.l2, r[2][k], fmaf(qw.l1, r[1][k], fmaf(qw.l0, r[0][k], 0.f)));
+            if constexpr (kComplex) vi[k] = fmaf(qw.l2, i[2][k], fmaf(qw.l1, i[1][k], fmaf(qw.l0, i[0][k], 0.f)));
         }
     }
 #pragma unroll
