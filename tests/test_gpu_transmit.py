@@ -136,3 +136,42 @@ def test_waves_in_a_batch_and_errors(device):
     assert "not both" in _lib.last_error()
     with pytest.raises(ValueError, match="unit vector"):
         transmit.PlaneWave(direction=(0.0, 0.0, 2.0), sound_speed_m_s=1540.0)
+
+
+@pytest.mark.parametrize("wave", [0, 4])
+def test_lateral_layouts_with_in_kernel_arrivals(device, wave):
+    """The lateral-group kernel evaluates the transmit arrival per voxel too: with ``tx_waves`` every
+    lateral layout equals, bit for bit, the general kernel given the same arrivals as an array; and
+    accumulation into ``out`` adds exactly the same values."""
+    from mach_amd import autotune
+
+    waves, specs = waves_and_specs()
+    waves, specs = [waves[wave]], [specs[wave]]
+    acq = phantoms.doppler(device=device, scale=0.25, n_frames=64)
+    scan = acq.scan_coords_m
+    arr = torch.as_tensor(wave_arrivals64(specs, scan.cpu().numpy()), device=device)
+    packed = transmit.pack(waves, device=device)
+    order = _hip_impl.voxel_order(scan)
+    p = acq.params
+    lib = _lib.load()
+
+    def run(out, layout, use_waves, accumulate=False):
+        args = _hip_impl.make_args(acq.channel_data, acq.rx_coords_m, scan, None if use_waves else arr, out, n_tx=1,
+                                   kind=_hip_impl._data_kind(acq.channel_data), f_number=p["f_number"],
+                                   rx_start_s=p["rx_start_s"], sampling_freq_hz=p["sampling_freq_hz"],
+                                   sound_speed_m_s=p["sound_speed_m_s"], modulation_freq_hz=p["modulation_freq_hz"],
+                                   tukey_alpha=p["tukey_alpha"], interp_type=p["interp_type"], accumulate=accumulate,
+                                   layout=layout, order=order, tx_waves=packed if use_waves else None)
+        _lib.check(lib.mach_beamform(ctypes.byref(args), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    ref = torch.zeros((acq.n_vox, 64), dtype=torch.complex64, device=device)
+    run(ref, autotune.encode(16, 4), False)
+    for lay in [x for x in autotune.candidates(64, band=False, lateral=True) if x & autotune.LATERAL_FLAG]:
+        got = torch.zeros_like(ref)
+        run(got, lay, True)
+        twice = got.clone()
+        run(twice, lay, True, accumulate=True)
+        torch.cuda.synchronize()
+        assert torch.abs(ref).max() > 0
+        assert torch.equal(got, ref), hex(lay)
+        assert torch.equal(twice, ref + ref), hex(lay)
