@@ -10,8 +10,9 @@
 // voxels and 5 rows serve 4, where the general kernel reads 6 and 12. Elements whose tap starts spread
 // wider than the window are read voxel by voxel.
 //
-// Everything per voxel is wave-uniform here (each voxel's setup is broadcast with v_readlane into
-// scalar registers), so the window position and each voxel's offset in it are scalar branches.
+// Everything per voxel is wave-uniform here: each voxel's tap index is read with v_readlane into a
+// scalar register, so the window position and each voxel's offset in it are scalar branches (its
+// weights arrive by ds_bpermute, off the VALU).
 // Each voxel and frame is summed element by element with the general kernel's float operations
 // (combine_rows), so the result is bit-identical to every other layout. Single-transmit launches only.
 #pragma once
@@ -128,7 +129,7 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
 #pragma unroll
                     for (int q = 0; q < VPL; ++q) {
                         if (i0[q] < 0) continue;
-                        const ElemCoef cf = elem_coef<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane);
+                        const ElemCoef cf = elem_weights<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane, i0[q]);
                         const int k = i0[q] - start;
                         if (k == 0) {
                             combine_window<kComplex, INTERP, FPL, 0, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
@@ -142,7 +143,7 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
 #pragma unroll
                     for (int q = 0; q < VPL; ++q) {
                         if (i0[q] < 0) continue;
-                        const ElemCoef cf = elem_coef<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane);
+                        const ElemCoef cf = elem_weights<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane, i0[q]);
                         const char *rq = r0 + (int64_t)i0[q] * step;
                         float tr[kN][FPL], ti[kN][FPL];
 #pragma unroll
