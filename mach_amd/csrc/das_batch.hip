@@ -45,10 +45,12 @@ int ilog2(int v) {
 // The plain kernel's layout when it is compiled into the batched kernels; otherwise the compiled
 // variant nearest to it in frames per voxel group (lanes x frames per lane, in octaves) whose frames
 // per lane do not exceed the wanted ones (whose divisibility and alignment have been checked). Ties go
-// to more frames per lane. Band layouts run as the general kernel's heuristic choice.
+// to more frames per lane. Band layouts run as the general kernel's heuristic choice, lateral layouts
+// as the 64-lane layout with their frames per lane.
 DasLayout batch_layout(const DasParams &p, int store, DasLayout want) {
     if (want.lanes_per_voxel == 0) want = choose_layout(p, store);
     want.chunks_per_lane = 1;  // the batched kernels walk one frame group per lane
+    want.voxels_per_lane = 1;  // and have no lateral groups: a lateral layout runs as its 64-lane layout
     if (layout_index(want) >= 0) return want;
     const int target = ilog2(want.lanes_per_voxel * want.frames_per_lane);
     int best = kNumBatchLayouts - 1, best_d = 1 << 30;

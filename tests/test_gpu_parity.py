@@ -414,3 +414,35 @@ def test_lateral_layout_rejected_for_several_transmits(device):
     with pytest.raises(RuntimeError, match="lateral"):
         _hip_impl.beamform_compound(d, r, v, a, out, 0.8, -1e-7, 20e6, 1540.0, 5e6, 0.5, 2,
                                     layout=autotune.encode_lateral(2, 2))
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2])
+@pytest.mark.parametrize("kind", ["iq", "rf"])
+@pytest.mark.parametrize("S", [1, 2, 3, 4, 6])
+def test_lateral_layouts_short_records(device, interp, kind, S):
+    """Lateral groups on records shorter than their sample window (the window is never used, or only
+    where it fits) and a voxel count that is no multiple of the group: bitwise equal to a general
+    layout and within 1e-5 of the oracle."""
+    from mach_amd import _hip_impl, autotune
+
+    rng = np.random.default_rng(100 * S + 10 * interp + (kind == "rf"))
+    E, F, V = 7, 64, 37
+    D, rx, vox, arr = random_case(rng, T=1, E=E, S=S, F=F, V=V, kind=kind)
+    tau = arr[0][:, None] + np.linalg.norm(vox[:, None, :].astype(np.float64) - rx[None], axis=-1) / 1540.0
+    t0 = float(np.median(tau) - 0.5 * S / 20e6)  # centres the taps on the short record
+    d, r, v, a = to_dev(D, rx, vox, arr, device=device)
+    fmod = 0.0 if kind == "rf" else 5e6
+    args = (2.0, t0, 20e6, 1540.0, fmod, 0.5, interp)
+    ref_gpu = torch.zeros((V, F), dtype=torch.float32 if kind == "rf" else torch.complex64, device=device)
+    _hip_impl.beamform_compound(d, r, v, a, ref_gpu, *args, layout=autotune.encode(32, 2))
+    for lay in [x for x in autotune.candidates(F, band=False, lateral=True) if x & autotune.LATERAL_FLAG]:
+        got = torch.zeros_like(ref_gpu)
+        _hip_impl.beamform_compound(d, r, v, a, got, *args, layout=lay)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref_gpu), hex(lay)
+    ref = das64(D, rx, vox, arr, f_number=2.0, rx_start_s=t0, sampling_freq_hz=20e6, sound_speed_m_s=1540.0,
+                modulation_freq_hz=fmod, tukey_alpha=0.5, interp_type=interp)
+    if np.abs(ref).max() > 0:
+        assert rel_l2(ref_gpu.cpu().numpy(), ref) <= TOL_FP32
+    else:
+        assert not torch.any(ref_gpu != 0)
