@@ -25,13 +25,44 @@
 namespace mach {
 namespace dev {
 
-// Combines window rows K .. K + 2 (clamped to the window; only the first kN are read) into one voxel.
-template <bool kComplex, int INTERP, int FPL, int K, int kW>
-__device__ __forceinline__ void combine_window(const float (&wr)[kW][FPL], const float (&wi)[kW][FPL],
-                                               const ElemCoef &cf, float (&acc_re)[FPL], float (&acc_im)[FPL]) {
-    constexpr int a = K < kW ? K : kW - 1, b = K + 1 < kW ? K + 1 : kW - 1, c = K + 2 < kW ? K + 2 : kW - 1;
-    const float *const rp[3] = {wr[a], wr[b], wr[c]}, *const ip[3] = {wi[a], wi[b], wi[c]};
-    combine_rows<kComplex, INTERP, FPL>(rp, ip, cf.frac, cf.qw, cf.wc, cf.ws, acc_re, acc_im);
+// N sample rows of FPL frames as loaded; FP16 pairs stay packed until they are combined (the
+// conversions then fold into mixed-precision FMAs), so a window of FP16 rows holds half the registers.
+template <int ST, int N, int FPL>
+struct Rows {
+    hvec<2 * FPL> h[N];              // FP16 storage
+    float re[N][FPL], im[N][FPL];  // float32 / complex64 storage
+
+    __device__ __forceinline__ void load(int r, const char *q) {
+        if constexpr (ST == kStoreC32H)
+            h[r] = *(const hvec<2 * FPL> *)q;
+        else
+            load_at<ST, FPL>(q, re[r], im[r]);
+    }
+    __device__ __forceinline__ float real(int r, int k) const {
+        if constexpr (ST == kStoreC32H) return (float)h[r][2 * k];
+        return re[r][k];
+    }
+    __device__ __forceinline__ float imag(int r, int k) const {
+        if constexpr (ST == kStoreC32H) return (float)h[r][2 * k + 1];
+        return im[r][k];
+    }
+};
+
+// Combines rows K .. K + 2 of w (clamped to w; only the first kN are read) into one voxel.
+template <int INTERP, int K, int ST, int N, int FPL>
+__device__ __forceinline__ void combine_window(const Rows<ST, N, FPL> &w, const ElemCoef &cf, float (&acc_re)[FPL],
+                                               float (&acc_im)[FPL]) {
+    constexpr int idx[3] = {K < N ? K : N - 1, K + 1 < N ? K + 1 : N - 1, K + 2 < N ? K + 2 : N - 1};
+    float r[3][FPL], i[3][FPL];
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+            r[n][k] = w.real(idx[n], k);
+            i[n][k] = w.imag(idx[n], k);
+        }
+    const float *const rp[3] = {r[0], r[1], r[2]}, *const ip[3] = {i[0], i[1], i[2]};
+    combine_rows<ST != kStoreF32, INTERP, FPL>(rp, ip, cf.frac, cf.qw, cf.wc, cf.ws, acc_re, acc_im);
 }
 
 template <int ST, int INTERP, int VPL, int FPL>
@@ -42,7 +73,6 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
     constexpr int kEC = 64 / VPL;              // elements per setup round (setup lanes per voxel)
     constexpr int kE = kElemBytes<ST>;
     constexpr int kGroup = 64 * FPL;  // frames per block
-    constexpr bool kComplex = ST != kStoreF32;
     constexpr bool kPre = INTERP == kQuadratic;  // the setup lane forms the Lagrange weights
     int tid = (int)threadIdx.x;
     asm volatile("" : "+v"(tid));
@@ -123,20 +153,20 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
                     // one window of kW rows, placed to stay inside the record
                     const int start = lo < (int)p.n_samples - kW ? lo : (int)p.n_samples - kW;
                     const char *rs = r0 + (int64_t)start * step;
-                    float wr[kW][FPL], wi[kW][FPL];
+                    Rows<ST, kW, FPL> w;
 #pragma unroll
-                    for (int r = 0; r < kW; ++r) load_at<ST, FPL>(rs + r * step + fo, wr[r], wi[r]);
+                    for (int r = 0; r < kW; ++r) w.load(r, rs + r * step + fo);
 #pragma unroll
                     for (int q = 0; q < VPL; ++q) {
                         if (i0[q] < 0) continue;
                         const ElemCoef cf = elem_weights<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane, i0[q]);
                         const int k = i0[q] - start;
                         if (k == 0) {
-                            combine_window<kComplex, INTERP, FPL, 0, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                            combine_window<INTERP, 0>(w, cf, acc_re[q], acc_im[q]);
                         } else if (kSpread == 1 || k == 1) {
-                            combine_window<kComplex, INTERP, FPL, 1, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                            combine_window<INTERP, 1>(w, cf, acc_re[q], acc_im[q]);
                         } else {
-                            combine_window<kComplex, INTERP, FPL, 2, kW>(wr, wi, cf, acc_re[q], acc_im[q]);
+                            combine_window<INTERP, 2>(w, cf, acc_re[q], acc_im[q]);
                         }
                     }
                 } else {
@@ -145,10 +175,10 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
                         if (i0[q] < 0) continue;
                         const ElemCoef cf = elem_weights<64, INTERP, kPre>(s, pw_own, q * kEC + j, lane, i0[q]);
                         const char *rq = r0 + (int64_t)i0[q] * step;
-                        float tr[kN][FPL], ti[kN][FPL];
+                        Rows<ST, kN, FPL> t;
 #pragma unroll
-                        for (int n = 0; n < kN; ++n) load_at<ST, FPL>(rq + n * step + fo, tr[n], ti[n]);
-                        combine_window<kComplex, INTERP, FPL, 0, kN>(tr, ti, cf, acc_re[q], acc_im[q]);
+                        for (int n = 0; n < kN; ++n) t.load(n, rq + n * step + fo);
+                        combine_window<INTERP, 0>(t, cf, acc_re[q], acc_im[q]);
                     }
                 }
             }
