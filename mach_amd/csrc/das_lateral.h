@@ -2,8 +2,9 @@
 // processing order (with the depth-major order: lateral neighbours at one depth), for 64 * FPL frames
 // (lane l holds frames l * FPL .. l * FPL + FPL - 1 of the block for every voxel of the group).
 //
-// Why: the general kernel's gathers are bound by the vector L1's delivery rate (one 64-byte line per
-// clock per CU): every voxel reads its own kN interpolation rows per element. Neighbouring voxels at
+// Why: on the 256-frame Doppler ensemble the general kernel keeps the vector-memory address unit (TA,
+// which feeds the L1) 80 % busy, because every voxel reads its own kN interpolation rows per element.
+// Neighbouring voxels at
 // one depth have nearly the same delay to an element (their first taps differ by at most a sample or
 // two), so the group's taps fall in one window of kN + kSpread rows. The wave loads that window once
 // and each voxel interpolates from its own rows of it: for quadratic interpolation 4 rows serve 2
