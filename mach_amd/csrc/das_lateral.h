@@ -4,12 +4,11 @@
 //
 // Why: on the 256-frame Doppler ensemble the general kernel keeps the vector-memory address unit (TA,
 // which feeds the L1) 80 % busy, because every voxel reads its own kN interpolation rows per element.
-// Neighbouring voxels at
-// one depth have nearly the same delay to an element (their first taps differ by at most a sample or
-// two), so the group's taps fall in one window of kN + kSpread rows. The wave loads that window once
-// and each voxel interpolates from its own rows of it: for quadratic interpolation 4 rows serve 2
-// voxels and 5 rows serve 4, where the general kernel reads 6 and 12. Elements whose tap starts spread
-// wider than the window are read voxel by voxel.
+// Neighbouring voxels at one depth have nearly the same delay to an element (their first taps differ
+// by at most a sample or two), so the group's taps fall in one window of kN + kSpread rows. The wave
+// loads that window once and each voxel interpolates from its own rows of it: for quadratic
+// interpolation 4 rows serve 2 voxels and 5 rows serve 4, where the general kernel reads 6 and 12.
+// Elements whose tap starts spread wider than the window are read voxel by voxel.
 //
 // Everything per voxel is wave-uniform here: each voxel's tap index is read with v_readlane into a
 // scalar register, so the window position and each voxel's offset in it are scalar branches (its
