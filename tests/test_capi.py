@@ -125,6 +125,19 @@ def test_layout_hint_is_validated():
     assert "layout" in _lib.last_error()
 
 
+@pytest.mark.parametrize("layout, n_tx", [(0x9402, 2),   # lateral groups sum a single transmit
+                                          (0xB402, 1),   # 8 voxels per lane is not compiled
+                                          (0x9401, 1),   # 1 frame per lane is not compiled
+                                          (0x9202, 1)])  # lateral groups take 64 lanes
+def test_lateral_layout_codes_are_validated(layout, n_tx):
+    lib = _lib.load()
+    a = _args(layout=layout, n_tx=n_tx)
+    a.out = 256  # non-NULL placeholder; validation fails before any device access
+    a.channel_data = a.rx_coords_m = a.scan_coords_m = a.tx_wave_arrivals_s = 256
+    assert lib.mach_beamform(ctypes.byref(a), None) == 1
+    assert "lateral" in _lib.last_error()
+
+
 def test_output_kind_and_peak_are_validated():
     lib = _lib.load()
     assert lib.mach_beamform(ctypes.byref(_args(output=7)), None) == 1
