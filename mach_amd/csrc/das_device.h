@@ -293,10 +293,13 @@ __device__ __forceinline__ void combine_rows(const float *const (&r)[3], const f
             vi[k] = i[0][k];
         }
     } else if constexpr (INTERP == kLinear) {
+        // The reference's lerp a + t (b - a) (vendor/helper_math.h:1150-1152), written as
+        // (a - t a) + t b: the same value up to rounding, exact at t = 0 and t = 1, and two FMAs that
+        // take FP16 taps straight from the packed halves (mixed-precision FMAs, no conversions).
 #pragma unroll
-        for (int k = 0; k < FPL; ++k) {  // lerp a + t (b - a) (vendor/helper_math.h:1150-1152)
-            vr[k] = fmaf(frac, r[1][k] - r[0][k], r[0][k]);
-            if constexpr (kComplex) vi[k] = fmaf(frac, i[1][k] - i[0][k], i[0][k]);
+        for (int k = 0; k < FPL; ++k) {
+            vr[k] = fmaf(frac, r[1][k], fmaf(-frac, r[0][k], r[0][k]));
+            if constexpr (kComplex) vi[k] = fmaf(frac, i[1][k], fmaf(-frac, i[0][k], i[0][k]));
         }
     } else {
         // The first product is written as fmaf(l0, a, 0), which rounds like l0 * a (a zero product
