@@ -95,14 +95,13 @@ inline DasLayout decode_layout(int32_t code) {
 //   volume, 2x4, 5 transmits (1 row)          50.5        39.5       37.5 (128)
 //   single plane wave, 4x1 (1 row)             0.057       0.054
 //   Doppler c64 signal, 32x2 (4 rows)          0.493       0.515
-//   75-angle compounding, 1x1 (1 row)          5.40        5.61
-// Launches of at most two rows with fewer transmits than a chunk take runs of blocks_x / 128 blocks
-// (a power of two in [8, 128]): each XCD then sweeps the depth range in ~16 bands, balanced, while a
-// run keeps neighbouring voxels, which share channel-data rows, in one L2. With more rows or many
-// transmits the L2 reuse of contiguous eighths wins.
+//   75-angle compounding, 1x1 (1 row)          4.50        4.01
+// Launches of at most two rows take runs of blocks_x / 128 blocks (a power of two in [8, 128]): each
+// XCD then sweeps the depth range in ~16 bands, balanced, while a run keeps neighbouring voxels, which
+// share channel-data rows, in one L2. With more rows the L2 reuse of contiguous eighths wins.
 inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_x, int64_t blocks_y) {
     if (p.xcd_group >= 0) return (uint32_t)p.xcd_group;
-    if (blocks_y > 2 || p.n_tx >= kTxChunk) return 0u;
+    if (blocks_y > 2) return 0u;
     uint32_t g = 8u;
     while (g < 128u && (int64_t)(2 * g) * 128 <= blocks_x) g *= 2u;
     return g;
