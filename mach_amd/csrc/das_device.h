@@ -62,14 +62,18 @@ __device__ __forceinline__ PairGeom pair_geom(const DasParams &p, double xv, dou
     if (h2 > R2) return g;  // outside the f-number aperture (kernel.cu:462)
     g.dic = sqrt(h2 + dz * dz) * p.inv_c;
     float w = 1.f;
-    if (p.apod) {
-        const double rr = (h2 == 0.0) ? 0.0 : sqrt(h2) * inv_R;
-        if (rr < 0.0 || rr > 1.0) {
+    // Tukey weight of r = |h| / R (kernel.cu:333-349; r = 0 when h = 0): 0 for r < 0 (a voxel behind
+    // the array, R < 0); 1 on the flat top r <= 1 - alpha, decided in float64 as
+    // h^2 <= (1 - alpha)^2 R^2; on the taper 0.5 - 0.5 cos(pi (1 - r) / alpha), formed as
+    // sin^2(pi/2 (1 - r) / alpha) (accurate near the window edge) in float32 with the hardware sine,
+    // which takes revolutions: (1 - r) / (4 alpha). The weight is continuous at both ends of the
+    // taper, so the float32 r changes it only by rounding. (r > 1 cannot occur: h^2 <= R^2 here.)
+    if (p.apod && h2 != 0.0) {
+        if (inv_R < 0.0) {
             w = 0.f;
-        } else if (!(rr <= 1.0 - p.alpha)) {
-            // 0.5 - 0.5 cos(2x) == sin(x)^2, accurate near the window edge
-            const float xh = (float)((1.0 - rr) * p.half_pi_over_alpha);
-            const float sn = sinf(xh);
+        } else if (!(h2 <= p.flat_r2 * R2)) {
+            const float rr = sqrtf((float)h2) * (float)inv_R;
+            const float sn = __builtin_amdgcn_sinf((1.f - rr) * p.quarter_over_alpha);
             w = sn * sn;
         }
     }

@@ -112,8 +112,8 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->t0 = (double)a->rx_start_s;
     p->fs = (double)a->sampling_freq_hz;
     p->fmod = (double)a->modulation_freq_hz;
-    p->alpha = (double)a->tukey_alpha;
-    p->half_pi_over_alpha = a->tukey_alpha > 0.0f ? (M_PI / 2.0) / (double)a->tukey_alpha : 0.0;
+    p->flat_r2 = (1.0 - (double)a->tukey_alpha) * (1.0 - (double)a->tukey_alpha);
+    p->quarter_over_alpha = a->tukey_alpha > 0.0f ? (float)(0.25 / (double)a->tukey_alpha) : 0.f;
     p->rotate = (a->data_type != MACH_DATA_F32 && a->modulation_freq_hz != 0.0f) ? 1 : 0;
     p->accumulate = a->accumulate ? 1 : 0;
     p->apod = a->tukey_alpha > 0.0f ? 1 : 0;  // kernel.cu:749
