@@ -108,14 +108,24 @@ def time_layout(run, layout: int, reps: int = 20, warmup: int = 3) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(run, n_frames: int, reps: int = 20, band: bool = True, lateral: bool = False):
-    """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms})."""
-    times = {}
-    for lay in candidates(n_frames, band=band, lateral=lateral):
-        try:
-            times[lay] = time_layout(run, lay, reps)
-        except RuntimeError:  # layout rejected for this request (alignment)
-            continue
+def tune(run, n_frames: int, reps: int = 20, band: bool = True, lateral: bool = False, rounds: int = 3):
+    """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms}).
+
+    Each candidate is timed in ``rounds`` passes over the list, alternately forwards and backwards, and
+    keeps its fastest pass: one short pass per candidate let clock ramps and noise decide between
+    layouts within a few percent of each other."""
+    times, rejected = {}, set()
+    cands = candidates(n_frames, band=band, lateral=lateral)
+    for r in range(rounds):
+        for lay in cands if r % 2 == 0 else cands[::-1]:
+            if lay in rejected:
+                continue
+            try:
+                t = time_layout(run, lay, reps)
+            except RuntimeError:  # layout rejected for this request (alignment)
+                rejected.add(lay)
+                continue
+            times[lay] = min(t, times.get(lay, t))
     best = min(times, key=times.get)
     return best, times
 

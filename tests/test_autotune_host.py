@@ -63,3 +63,26 @@ def test_table_roundtrip_and_alignment_guard(tmp_path):
         assert autotune.layout_for(1, 1, 1, 128, 8, 4096, data_ptr=8, out_ptr=256) == 0  # misaligned for 4 frames
     finally:
         autotune._TABLE = old
+
+
+def test_tune_alternates_passes_keeps_fastest_and_skips_rejected(monkeypatch):
+    cands = autotune.candidates(8)
+    rejected = cands[1]
+    calls = []
+    # each layout's scripted times per pass; the first candidate is slow on its first (cold) pass only
+    script = {lay: [1.0 + 0.01 * i] * 3 for i, lay in enumerate(cands)}
+    script[cands[0]] = [9.0, 0.5, 0.6]
+
+    def fake_time(run, lay, reps):
+        calls.append(lay)
+        if lay == rejected:
+            raise RuntimeError("layout rejected")
+        return script[lay][sum(1 for c in calls if c == lay) - 1]
+
+    monkeypatch.setattr(autotune, "time_layout", fake_time)
+    best, times = autotune.tune(lambda lay: None, 8, reps=1, band=False, rounds=3)
+    assert best == cands[0] and times[cands[0]] == 0.5
+    assert rejected not in times and calls.count(rejected) == 1
+    n = len(cands)
+    assert calls[:n] == cands  # pass 1 forwards
+    assert calls[n:2 * n - 1] == [c for c in cands[::-1] if c != rejected]  # pass 2 backwards
