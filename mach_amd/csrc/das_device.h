@@ -106,27 +106,30 @@ __device__ __forceinline__ double wave_arrival(const WaveDesc &w, double x, doub
 template <int INTERP>
 __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeom &g, double Tv) {
     PairSetup r{-1, 0.f, 0.f, 0.f};
+    if (g.w == 0.f) return r;  // outside the aperture: often the whole wave, which then skips the rest
     const double tau = Tv + g.dic;
-    if (tau < 0.0 || g.w == 0.f) return r;
     const double s = (tau - p.t0) * p.fs;
     const double ns = (double)p.n_samples;
+    // The other skips (tau < 0, taps outside the record) are one select at the end: as early returns,
+    // each nested exit re-initialised the result under its own exec mask and held that mask in scalar
+    // registers, which spilled in the multi-transmit kernels. Values computed for a skipped pair are
+    // discarded by the select.
+    bool ok = !(tau < 0.0);
     int i0;
     float frac = 0.f;
     if constexpr (INTERP == kNearest) {
-        if (s < -0.5 || s > ns - 0.5) return r;
+        ok = ok && !(s < -0.5 || s > ns - 0.5);
         const double k = rint(s);  // round half to even
         i0 = (int)fmin(k, ns - 1.0);  // s in (n-1, n-0.5] would round to n: clamp to the last sample
     } else if constexpr (INTERP == kLinear) {
-        if (s < 0.0 || s > ns - 1.0) return r;
+        ok = ok && !(s < 0.0 || s > ns - 1.0);
         const double fl = floor(s);
-        i0 = (int)fl;
-        frac = (float)(s - fl);
-        if (i0 == p.n_samples - 1 && p.n_samples >= 2) {  // s == n-1 exactly: taps (n-2, n-1), t = 1
-            i0 -= 1;
-            frac = 1.f;
-        }
+        const int i = (int)fl;
+        const bool last = i == p.n_samples - 1 && p.n_samples >= 2;  // s == n-1 exactly: taps (n-2, n-1), t = 1
+        i0 = last ? i - 1 : i;
+        frac = last ? 1.f : (float)(s - fl);
     } else {
-        if (s < 1.0 || s > ns - 2.0) return r;
+        ok = ok && !(s < 1.0 || s > ns - 2.0);
         const double c = rint(s);
         i0 = (int)c - 1;
         frac = (float)(s - c);
@@ -142,10 +145,10 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
         wc = g.w * cs;
         ws = g.w * sn;
     }
-    r.i0 = i0;
-    r.frac = frac;
-    r.wc = wc;
-    r.ws = ws;
+    r.i0 = ok ? i0 : -1;
+    r.frac = ok ? frac : 0.f;
+    r.wc = ok ? wc : 0.f;
+    r.ws = ok ? ws : 0.f;
     return r;
 }
 
