@@ -86,3 +86,22 @@ def test_tune_alternates_passes_keeps_fastest_and_skips_rejected(monkeypatch):
     n = len(cands)
     assert calls[:n] == cands  # pass 1 forwards
     assert calls[n:2 * n - 1] == [c for c in cands[::-1] if c != rejected]  # pass 2 backwards
+
+
+def test_shipped_table_entries_are_candidates_of_their_shape():
+    """Every layout in the shipped gfx950 table is one the tuner could have chosen for its key: a
+    candidate for the key's frame count, with band layouts only for single-transmit signal requests
+    and lateral layouts only for single-transmit requests."""
+    import re
+
+    with open(autotune.DEFAULT_TABLE) as fh:
+        data = json.load(fh)
+    assert data["arch"] == "gfx950" and data["entries"]
+    for key, entry in data["entries"].items():
+        m = re.fullmatch(r"d(\d+):i(\d):t(\d+):e(\d+):f(\d+):v(\d+)(?::o(\d))?", key)
+        assert m, key
+        n_tx, n_frames, output = int(m.group(3)), int(m.group(5)), int(m.group(7) or 0)
+        single = n_tx == 1
+        cands = autotune.candidates(n_frames, band=single and output == 0, lateral=single)
+        assert int(entry["layout"]) in cands, (key, hex(int(entry["layout"])))
+        assert entry["layout_hex"] == hex(int(entry["layout"]))
