@@ -116,8 +116,12 @@ typedef struct mach_beamform_args {
                                          holds its frames of 2 or 4 voxels consecutive in
                                          voxel_order, read from shared sample windows (single
                                          transmit only). Every layout gives bit-identical
-                                         results for single-transmit requests; general-kernel
-                                         layouts also for several transmits.                   */
+                                         signal and envelope outputs for single-transmit
+                                         requests; general-kernel layouts also for several
+                                         transmits. The power image is not layout-independent:
+                                         each lane sums |y|^2 over its own frames before the
+                                         lanes of a voxel are added, so its rounding depends on
+                                         lanes per voxel and frames per lane.                   */
     const int32_t *voxel_order;       /* optional device [n_vox] permutation of 0..n_vox-1: the
                                          order voxels are processed in (NULL = 0, 1, 2, ...).
                                          Results still land at out[v]; a depth-major order
@@ -155,7 +159,10 @@ int mach_beamform_f32(const float *channel_data, const float *rx_coords_m, const
                       float tukey_alpha, int32_t interp_type, void *stream);
 
 /* General entry (transmit compounding, FP16 IQ storage, overwrite or accumulate). Replaces
- * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209. */
+ * src/mach/experimental.py:23-106 over src/mach/kernel.cu:1033-1209.
+ * A request with no term to sum (n_vox, n_frames, n_tx or n_rx equal to 0) launches no kernel and
+ * reads none of its inputs, which may then be NULL: overwrite mode writes zeros to `out` (n_vox values
+ * for the power image), accumulate mode leaves it unchanged, and `peak` is set to 0. */
 int mach_beamform(const mach_beamform_args *args, void *stream);
 
 /* Materialises the transmit arrivals of `n_tx` device wavefronts for `n_vox` device voxel
@@ -198,7 +205,9 @@ typedef struct mach_batch mach_batch;
 int mach_batch_create(uint32_t max_requests, int32_t use_graph, mach_batch **batch);
 
 /* Enqueues all `n_requests` requests on `stream` as one submission. The band layouts run as the
- * general kernel here. */
+ * general kernel here. All or nothing: every request is validated before anything is enqueued, and
+ * the first invalid one fails the call (mach_last_error() starts with "request <index>: "), so the
+ * caller can drop it and resubmit the others. */
 int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uint32_t n_requests,
                       void *stream);
 

@@ -177,23 +177,71 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
     return MACH_OK;
 }
 
+// True when a validated request has no term to sum: no voxels, no frames, no transmits or no receive
+// elements. Such a request launches no delay-and-sum kernel and reads none of its inputs (validation
+// lets them be NULL then); the power image still has one (zero) value per voxel.
+static bool empty_sum(const mach::DasParams &p) {
+    return p.n_vox == 0 || p.n_frames == 0 || p.n_tx == 0 || p.n_rx == 0;
+}
+
+// Result of an empty sum: overwrite mode writes zeros to every output value, accumulate mode leaves
+// `out` unchanged; `peak` (if given) is set to 0.
+static hipError_t enqueue_empty_sum(const mach::DasParams &p, int data_type, hipStream_t stream) {
+    hipError_t e = hipSuccess;
+    uint64_t n_out = 0;  // float32 values of `out`
+    if (p.output == mach::kOutPower)
+        n_out = (uint64_t)p.n_vox;
+    else
+        n_out = (uint64_t)p.n_vox * (uint64_t)p.n_frames *
+                ((p.output == mach::kOutSignal && data_type != MACH_DATA_F32) ? 2u : 1u);
+    if (!p.accumulate && n_out > 0) e = mach::launch_fill((float *)p.out, n_out, 0.f, stream);
+    if (e == hipSuccess && p.peak != nullptr) e = mach::launch_fill(p.peak, 1, 0.f, stream);
+    return e;
+}
+
+// Validates every request of a batch and resolves its kernel variant into caller-provided arrays of
+// n entries. All-or-nothing: the first invalid request fails the whole batch (its index prefixes the
+// message) before anything is enqueued, so a caller can resubmit the valid requests.
+static int build_batch(const mach_beamform_args *requests, uint32_t n, mach::DasParams *params, int *stores,
+                       int *interps, mach::DasLayout *layouts) {
+    for (uint32_t i = 0; i < n; ++i) {
+        int st = validate_and_build(&requests[i], &params[i]);
+        if (st == MACH_OK && params[i].output != mach::kOutSignal)
+            st = fail(MACH_ERR_UNSUPPORTED, "Error: batches run the signal output only");
+        if (st == MACH_OK && !params[i].bytes32)
+            st = fail(MACH_ERR_UNSUPPORTED, "Error: channel blocks of 4 GiB or more (n_samples * n_frames * element "
+                                            "bytes) run through mach_beamform only");
+        Plan plan{false, 0, mach::DasLayout{1, 1}};
+        if (st == MACH_OK && !empty_sum(params[i])) st = resolve_plan(&requests[i], params[i], &plan);
+        if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
+        layouts[i] = mach::batch_layout(params[i], requests[i].data_type, plan.layout);
+        stores[i] = requests[i].data_type;
+        interps[i] = requests[i].interp_type;
+    }
+    return MACH_OK;
+}
+
+// Empty sums of a validated batch get their fill (enqueue_empty_sum) and are taken out of the work
+// table (n_vox = 0), so the persistent kernels never read their (possibly NULL) inputs.
+static hipError_t enqueue_empty_sums(const mach_beamform_args *requests, mach::DasParams *params, uint32_t n,
+                                     hipStream_t stream) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!empty_sum(params[i])) continue;
+        const hipError_t e = enqueue_empty_sum(params[i], requests[i].data_type, stream);
+        if (e != hipSuccess) return e;
+        params[i].n_vox = 0;
+    }
+    return hipSuccess;
+}
+
 int mach_beamform(const mach_beamform_args *args, void *stream) {
     g_last_error.clear();
     mach::DasParams p;
     int st = validate_and_build(args, &p);
     if (st != MACH_OK) return st;
-    if (p.n_vox > 0 && p.n_frames == 0 && p.output == mach::kOutPower) {  // sum over no frames
-        hipError_t e = hipSuccess;
-        if (!p.accumulate) e = mach::launch_fill((float *)p.out, (uint64_t)p.n_vox, 0.f, (hipStream_t)stream);
-        if (e == hipSuccess && p.peak != nullptr) e = mach::launch_fill(p.peak, 1, 0.f, (hipStream_t)stream);
-        return e == hipSuccess ? MACH_OK : hip_fail(e, "beamform power fill");
-    }
-    if (p.n_vox == 0 || p.n_frames == 0) {
-        if (p.peak != nullptr) {
-            const hipError_t e = mach::launch_fill(p.peak, 1, 0.f, (hipStream_t)stream);
-            if (e != hipSuccess) return hip_fail(e, "beamform peak fill");
-        }
-        return MACH_OK;
+    if (empty_sum(p)) {
+        const hipError_t e = enqueue_empty_sum(p, args->data_type, (hipStream_t)stream);
+        return e == hipSuccess ? MACH_OK : hip_fail(e, "beamform empty-sum fill");
     }
     Plan plan;
     st = resolve_plan(args, p, &plan);
@@ -229,23 +277,12 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
     std::vector<mach::DasParams> params(n_requests);
     std::vector<int> stores(n_requests), interps(n_requests);
     std::vector<mach::DasLayout> layouts(n_requests);
-    for (uint32_t i = 0; i < n_requests; ++i) {
-        int st = validate_and_build(&requests[i], &params[i]);
-        if (st == MACH_OK && params[i].output != mach::kOutSignal)
-            st = fail(MACH_ERR_UNSUPPORTED, "Error: batches run the signal output only");
-        if (st == MACH_OK && !params[i].bytes32)
-            st = fail(MACH_ERR_UNSUPPORTED, "Error: channel blocks of 4 GiB or more (n_samples * n_frames * element "
-                                            "bytes) run through mach_beamform only");
-        Plan plan{false, 0, mach::DasLayout{1, 1}};
-        if (st == MACH_OK && params[i].n_vox > 0 && params[i].n_frames > 0) st = resolve_plan(&requests[i], params[i], &plan);
-        if (st != MACH_OK) return fail(st, "request " + std::to_string(i) + ": " + g_last_error);
-        layouts[i] = plan.layout;
-        layouts[i] = mach::batch_layout(params[i], requests[i].data_type, layouts[i]);
-        stores[i] = requests[i].data_type;
-        interps[i] = requests[i].interp_type;
-    }
-    const hipError_t e = mach::batch_submit(batch->state, params.data(), stores.data(), interps.data(),
-                                            layouts.data(), n_requests, (hipStream_t)stream);
+    const int st = build_batch(requests, n_requests, params.data(), stores.data(), interps.data(), layouts.data());
+    if (st != MACH_OK) return st;
+    hipError_t e = enqueue_empty_sums(requests, params.data(), n_requests, (hipStream_t)stream);
+    if (e == hipSuccess)
+        e = mach::batch_submit(batch->state, params.data(), stores.data(), interps.data(), layouts.data(), n_requests,
+                               (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_batch_submit");
     return MACH_OK;
 }

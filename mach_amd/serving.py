@@ -105,8 +105,13 @@ class DeviceRing:
         return sum(e[1] for e in self.live)
 
     def alloc(self, nbytes: int):
-        """Reserve ``nbytes``; returns (start, size) or None if it does not fit now."""
+        """Reserve ``nbytes``; returns (start, size) or None if it does not fit now. A zero-byte
+        reservation returns the slot (0, 0), which holds no space and is never live (a live empty
+        entry at ``head`` would make the ring look full until it is released)."""
         n = (int(nbytes) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        if n == 0:
+            self._undo = None
+            return (0, 0)
         if n > self.capacity:
             raise ValueError(f"request of {nbytes} bytes exceeds the ring capacity {self.capacity}")
         if not self.live:
@@ -131,8 +136,11 @@ class DeviceRing:
 
     def undo_last(self):
         """Release the most recent allocation (admission of a request that did not fully fit)."""
+        if self._undo is None:  # the last reservation was of zero bytes
+            return
         self.live.pop()
         self.head = self._undo if self.live else 0
+        self._undo = None
 
     def _reclaim(self):
         while self.live and self.live[0][2]:
@@ -142,6 +150,8 @@ class DeviceRing:
 
     def free(self, slot):
         """Release the allocation ``slot`` (as returned by ``alloc``), in any order."""
+        if slot[1] == 0:  # zero-byte slots hold nothing
+            return
         for e in self.live:
             if e[0] == slot[0] and e[1] == slot[1] and not e[2]:
                 e[2] = True

@@ -90,6 +90,20 @@ def test_empty_problem_is_a_no_op():
     assert lib.mach_beamform(ctypes.byref(_args(n_frames=0)), None) == 0
 
 
+@pytest.mark.parametrize("n_tx, n_rx", [(1, 0), (0, 4), (5, 0)])
+def test_empty_sum_in_accumulate_mode_touches_nothing(n_tx, n_rx):
+    """No transmits or no receive elements: validation lets the arrivals (and, with no elements, the
+    coordinates and channel data) be NULL, so the call must not launch the kernel, which reads the
+    arrivals of every voxel. In accumulate mode nothing is written either: MACH_OK with no device work
+    (no GPU here, so any HIP call would fail with a status)."""
+    lib = _lib.load()
+    a = _args(n_tx=n_tx, n_rx=n_rx, accumulate=1)
+    a.out = a.scan_coords_m = 256  # non-NULL placeholders, never dereferenced
+    assert a.tx_wave_arrivals_s is None and a.tx_waves is None and a.channel_data is None
+    assert lib.mach_beamform(ctypes.byref(a), None) == 0, _lib.last_error()
+    assert _lib.last_error() == ""
+
+
 def test_device_query_reports_status():
     lib = _lib.load()
     buf = ctypes.create_string_buffer(64)

@@ -44,6 +44,22 @@ def test_ring_undo_and_limits():
     assert view.shape == (4, 32)
 
 
+def test_zero_byte_reservations_hold_nothing():
+    ring = DeviceRing(1024, "cpu")
+    z = ring.alloc(0)
+    assert z == (0, 0) and ring.used == 0 and not ring.live
+    assert ring.alloc(1024) == (0, 1024)  # the empty slot does not make the ring look full
+    ring.undo_last()
+    assert ring.used == 0
+    z = ring.alloc(0)
+    ring.undo_last()  # undoing a zero-byte reservation releases nothing else
+    a = ring.alloc(512)
+    assert a == (0, 512)
+    ring.free(z)  # zero-byte slots free as no-ops, in any order
+    ring.free(a)
+    assert ring.used == 0 and ring.alloc(1024) == (0, 1024)
+
+
 def test_fusion_groups_by_geometry_and_frame_cap():
     import torch
 
