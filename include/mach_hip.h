@@ -214,6 +214,22 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
 /* Waits for the batch's pending work and frees it. */
 int mach_batch_destroy(mach_batch *batch);
 
+/* Requests one mach_batch_submit_noalloc call takes at most. */
+#define MACH_NOALLOC_MAX_REQUESTS 64
+
+/* Allocation-free variant of mach_batch_submit, with no counterpart in the reference: for callers that
+ * cannot allocate (code under stream or graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It needs no mach_batch object and allocates nothing, on the host or the device: the
+ * descriptor table and work counters are a fixed device array of the library (about 14 KiB, held by
+ * each GPU the library runs on until the process ends), written from kernel arguments, and the same
+ * persistent kernels run the requests without graph replay. Results equal mach_batch_submit's and
+ * mach_beamform's on each request. At most MACH_NOALLOC_MAX_REQUESTS requests (MACH_ERR_INVALID_ARGUMENT
+ * above that); validation is all or nothing, as for mach_batch_submit. Every call on one GPU shares its
+ * table: calls on one stream are ordered, but calls that overlap on one GPU (other streams, other
+ * threads) race. The request array may be reused as soon as the call returns; the arrays it points to
+ * must stay valid until the work has completed on `stream`. Only error paths allocate (the message). */
+int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_requests, void *stream);
+
 /* Message of the last failure on this thread ("" if none). Replaces the text of the reference's
  * std::runtime_error messages (src/mach/kernel.cu:740-769, 926-950). */
 const char *mach_last_error(void);

@@ -76,6 +76,7 @@ SIGNATURES = {
     "mach_batch_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BeamformArgs), ctypes.c_uint32,
                                          ctypes.c_void_p]),
     "mach_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "mach_batch_submit_noalloc": (ctypes.c_int, [ctypes.POINTER(BeamformArgs), ctypes.c_uint32, ctypes.c_void_p]),
     "mach_to_db": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int32,
                                   ctypes.c_void_p]),
     "mach_analytic_signal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,

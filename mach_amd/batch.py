@@ -86,6 +86,18 @@ def run_plain(req: Request, stream=None) -> None:
     _lib.check(_hip_impl._LIB.mach_beamform(ctypes.byref(a), ctypes.c_void_p(s)))
 
 
+def submit_noalloc(requests, stream=None) -> None:
+    """All requests as one persistent launch on ``stream`` through ``mach_batch_submit_noalloc``: no
+    batch object, no allocation (at most 64 requests; calls overlapping on one GPU race). Tensors must
+    stay alive until that work completes."""
+    n = len(requests)
+    arr = (_lib.BeamformArgs * max(n, 1))()
+    for i, r in enumerate(requests):
+        arr[i] = r.args()
+    s = torch.cuda.current_stream().cuda_stream if stream is None else stream.cuda_stream
+    _lib.check(_hip_impl._LIB.mach_batch_submit_noalloc(arr, n, ctypes.c_void_p(s)), "mach_batch_submit_noalloc")
+
+
 class BatchRunner:
     """Owns a ``mach_batch`` (two descriptor slots, graph replay)."""
 

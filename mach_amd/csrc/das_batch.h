@@ -22,4 +22,11 @@ bool batch_layout_supported(DasLayout l);
 hipError_t batch_submit(BatchState *b, const DasParams *params, const int *stores, const int *interps,
                         const DasLayout *layouts, uint32_t n, hipStream_t stream);
 
+// Requests one allocation-free submission takes (the size of its static device table).
+constexpr uint32_t kNoAllocMaxRequests = 64;
+// batch_submit without a BatchState: the table and counters are the device's static scratch array, the
+// table is written by kernels from their arguments, and nothing is allocated (n <= kNoAllocMaxRequests).
+hipError_t batch_submit_noalloc(const DasParams *params, const int *stores, const int *interps,
+                                const DasLayout *layouts, uint32_t n, hipStream_t stream);
+
 }  // namespace mach

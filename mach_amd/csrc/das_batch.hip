@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "das_batch.h"
@@ -89,7 +91,7 @@ struct BatchState {
 
 namespace {
 
-size_t items_offset() { return (sizeof(BatchHeader) + 255) & ~size_t(255); }
+constexpr size_t items_offset() { return (sizeof(BatchHeader) + 255) & ~size_t(255); }
 
 // `sections` is a bit mask of the sections to launch: both when captured into a graph (the graph is
 // replayed for every batch; an empty section's workgroups exit at once), only non-empty ones otherwise.
@@ -165,19 +167,15 @@ void batch_destroy(BatchState *b) {
 
 uint32_t batch_capacity(const BatchState *b) { return b->max_items; }
 
-hipError_t batch_submit(BatchState *b, const DasParams *params, const int *stores, const int *interps,
-                        const DasLayout *layouts, uint32_t n, hipStream_t stream) {
-    BatchSlot &s = b->slot[b->next];
-    b->next ^= 1;
-    if (s.pending) {
-        const hipError_t e = hipEventSynchronize(s.done);  // previous use of this slot has finished
-        if (e != hipSuccess) return e;
-        s.pending = false;
-    }
-    BatchHeader *hdr = (BatchHeader *)s.h_buf;
-    BatchItem *items = (BatchItem *)((char *)s.h_buf + items_offset());
+namespace {
+
+// Fills the descriptor table (header and items, the items of each section consecutive) for the n
+// requests; requests with no voxels or frames get no item. Returns the non-empty sections as a bit mask
+// in *sections and the number of items written in *n_items.
+hipError_t pack_table(const DasParams *params, const int *stores, const int *interps, const DasLayout *layouts,
+                      uint32_t n, BatchHeader *hdr, BatchItem *items, unsigned *sections, uint32_t *n_items) {
     uint32_t kept = 0;
-    unsigned sections = 0;
+    *sections = 0;
     for (int sec = 0; sec < kSections; ++sec) {
         hdr->first_item[sec] = kept;
         uint64_t total = 0;
@@ -204,10 +202,30 @@ hipError_t batch_submit(BatchState *b, const DasParams *params, const int *store
         }
         hdr->n_items[sec] = kept - hdr->first_item[sec];
         hdr->total_blocks[sec] = (uint32_t)total;
-        if (total > 0) sections |= 1u << sec;
+        if (total > 0) *sections |= 1u << sec;
     }
+    *n_items = kept;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t batch_submit(BatchState *b, const DasParams *params, const int *stores, const int *interps,
+                        const DasLayout *layouts, uint32_t n, hipStream_t stream) {
+    BatchSlot &s = b->slot[b->next];
+    b->next ^= 1;
+    if (s.pending) {
+        const hipError_t e = hipEventSynchronize(s.done);  // previous use of this slot has finished
+        if (e != hipSuccess) return e;
+        s.pending = false;
+    }
+    BatchHeader *hdr = (BatchHeader *)s.h_buf;
+    BatchItem *items = (BatchItem *)((char *)s.h_buf + items_offset());
+    unsigned sections = 0;
+    uint32_t kept = 0;
+    hipError_t e = pack_table(params, stores, interps, layouts, n, hdr, items, &sections, &kept);
+    if (e != hipSuccess) return e;
     if (sections == 0) return hipSuccess;
-    hipError_t e;
     if (b->use_graph) {
         if (s.exec == nullptr) {
             e = capture(b, s);
@@ -220,6 +238,124 @@ hipError_t batch_submit(BatchState *b, const DasParams *params, const int *store
     if (e != hipSuccess) return e;
     e = hipEventRecord(s.done, stream);
     if (e == hipSuccess) s.pending = true;
+    return e;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Allocation-free batches (mach_batch_submit_noalloc). The descriptor table and the two work counters
+// live in `scratch`, a file-scope device array of fixed size: each GPU holds its own copy from module
+// load until the process ends. Host code never uses the array's name as an address (on the host it is
+// a stand-in symbol): the device address comes from hipGetSymbolAddress, once per device, at the
+// device's first call. The table reaches the device through kernel arguments (write_table), which the
+// runtime copies at launch, so the call needs no pinned staging and no allocation, and the caller may
+// reuse its request array as soon as the call returns. Every call on one GPU shares that GPU's array:
+// calls on one stream are ordered, calls that overlap on one GPU (other streams, other threads) race.
+
+constexpr size_t kNoAllocCountersOffset = 128;  // two uint32 work counters after the 32-byte header
+static_assert(sizeof(BatchHeader) <= kNoAllocCountersOffset, "header overlaps the counters");
+static_assert(kNoAllocCountersOffset + kSections * sizeof(uint32_t) <= items_offset(), "counters overlap items");
+constexpr size_t kNoAllocBytes = items_offset() + (size_t)kNoAllocMaxRequests * sizeof(BatchItem);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Items one write_table launch carries in its arguments (kept under 4 KiB of kernel arguments).
+constexpr uint32_t kItemsPerWrite = 14;
+
+struct TableWrite {
+    BatchHeader header;
+    uint32_t first;  // table index of items[0]
+    uint32_t count;  // items to write
+    uint32_t reset;  // 1: also write the header and zero the work counters
+    uint32_t pad;
+    BatchItem items[kItemsPerWrite];
+};
+static_assert(sizeof(TableWrite) <= 4096, "table write exceeds 4 KiB of kernel arguments");
+static_assert(sizeof(BatchItem) % 4 == 0 && sizeof(BatchHeader) % 4 == 0, "table words");
+
+// Copies `w` (kernel arguments) into the table at `table` word by word.
+__global__ __launch_bounds__(256) void write_table(const TableWrite w, unsigned char *table) {
+    const uint32_t tid = threadIdx.x;
+    constexpr uint32_t kItemWords = sizeof(BatchItem) / 4;
+    const uint32_t n_words = w.count * kItemWords;
+    uint32_t *dst = (uint32_t *)(table + items_offset()) + (size_t)w.first * kItemWords;
+    const uint32_t *src = (const uint32_t *)w.items;
+    for (uint32_t i = tid; i < n_words; i += blockDim.x) dst[i] = src[i];
+    if (w.reset) {
+        constexpr uint32_t kHeaderWords = sizeof(BatchHeader) / 4;
+        if (tid < kHeaderWords) ((uint32_t *)table)[tid] = ((const uint32_t *)&w.header)[tid];
+        if (tid < kSections) ((uint32_t *)(table + kNoAllocCountersOffset))[tid] = 0u;
+    }
+}
+
+// Per-device state of the allocation-free path, set up at the device's first call.
+struct NoAllocDevice {
+    std::atomic<int> ready{0};
+    unsigned char *table = nullptr;  // device address of this device's `scratch`
+    int grid[kSections] = {0, 0};    // persistent grid per section
+};
+constexpr int kNoAllocMaxDevices = 64;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+std::mutex g_noalloc_mutex;
+
+hipError_t noalloc_device(NoAllocDevice **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    NoAllocDevice &d = g_noalloc[dev];
+    if (!d.ready.load(std::memory_order_acquire)) {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        if (!d.ready.load(std::memory_order_relaxed)) {
+            void *ptr = nullptr;
+            e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // this device's copy of the array
+            if (e != hipSuccess) return e;
+            int cus = 0, per_cu[kSections] = {0, 0};
+            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e == hipSuccess) e = section_occupancy(0, &per_cu[0]);
+            if (e == hipSuccess) e = section_occupancy(1, &per_cu[1]);
+            if (e != hipSuccess) return e;
+            d.table = (unsigned char *)ptr;
+            for (int i = 0; i < kSections; ++i) d.grid[i] = (per_cu[i] > 0 ? per_cu[i] : 1) * (cus > 0 ? cus : 1);
+            d.ready.store(1, std::memory_order_release);
+        }
+    }
+    *out = &d;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t batch_submit_noalloc(const DasParams *params, const int *stores, const int *interps,
+                                const DasLayout *layouts, uint32_t n, hipStream_t stream) {
+    if (n > kNoAllocMaxRequests) return hipErrorInvalidValue;
+    BatchHeader hdr{};
+    BatchItem items[kNoAllocMaxRequests];
+    unsigned sections = 0;
+    uint32_t kept = 0;
+    hipError_t e = pack_table(params, stores, interps, layouts, n, &hdr, items, &sections, &kept);
+    if (e != hipSuccess || sections == 0) return e;  // nothing to run: no device state is touched
+    NoAllocDevice *d = nullptr;
+    e = noalloc_device(&d);
+    if (e != hipSuccess) return e;
+    TableWrite w;
+    for (uint32_t first = 0; first < kept; first += kItemsPerWrite) {
+        std::memset(&w, 0, sizeof(w));
+        w.header = hdr;
+        w.first = first;
+        w.count = kept - first < kItemsPerWrite ? kept - first : kItemsPerWrite;
+        w.reset = first == 0 ? 1u : 0u;
+        std::memcpy(w.items, items + first, (size_t)w.count * sizeof(BatchItem));
+        hipLaunchKernelGGL(write_table, dim3(1), dim3(256), 0, stream, w, d->table);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const BatchHeader *dh = (const BatchHeader *)d->table;
+    const BatchItem *di = (const BatchItem *)(d->table + items_offset());
+    uint32_t *dc = (uint32_t *)(d->table + kNoAllocCountersOffset);
+    if (sections & 1u) e = launch_section0((unsigned)d->grid[0], stream, dh, di, dc);
+    if (e == hipSuccess && (sections & 2u)) e = launch_section1((unsigned)d->grid[1], stream, dh, di, dc);
     return e;
 }
 

@@ -287,6 +287,28 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
     return MACH_OK;
 }
 
+int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_requests, void *stream) {
+    g_last_error.clear();
+    static_assert(MACH_NOALLOC_MAX_REQUESTS == mach::kNoAllocMaxRequests, "allocation-free table size");
+    if (n_requests > MACH_NOALLOC_MAX_REQUESTS)
+        return fail(MACH_ERR_INVALID_ARGUMENT, "Error: " + std::to_string(n_requests) +
+                                                   " requests exceed the allocation-free maximum of " +
+                                                   std::to_string(MACH_NOALLOC_MAX_REQUESTS));
+    if (n_requests > 0 && requests == nullptr) return fail(MACH_ERR_INVALID_ARGUMENT, "Error: requests is NULL");
+    if (n_requests == 0) return MACH_OK;
+    // fixed-size arrays on the stack: nothing is allocated on the success path
+    mach::DasParams params[MACH_NOALLOC_MAX_REQUESTS];
+    int stores[MACH_NOALLOC_MAX_REQUESTS], interps[MACH_NOALLOC_MAX_REQUESTS];
+    mach::DasLayout layouts[MACH_NOALLOC_MAX_REQUESTS];
+    const int st = build_batch(requests, n_requests, params, stores, interps, layouts);
+    if (st != MACH_OK) return st;
+    hipError_t e = enqueue_empty_sums(requests, params, n_requests, (hipStream_t)stream);
+    if (e == hipSuccess)
+        e = mach::batch_submit_noalloc(params, stores, interps, layouts, n_requests, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mach_batch_submit_noalloc");
+    return MACH_OK;
+}
+
 int mach_batch_destroy(mach_batch *batch) {
     g_last_error.clear();
     if (batch == nullptr) return MACH_OK;

@@ -181,3 +181,20 @@ def test_arrivals_and_waves_are_exclusive():
     assert "not both" in _lib.last_error()
     assert lib.mach_tx_arrivals(None, 0, None, 5, None, None) == 0  # nothing to compute
     assert lib.mach_tx_arrivals(None, 2, None, 5, None, None) == 1
+
+
+def test_noalloc_batch_rejects_before_device_work():
+    """mach_batch_submit_noalloc: the size limit, NULL requests and invalid requests are refused before
+    anything touches the device; an empty or all-empty submission is a no-op."""
+    lib = _lib.load()
+    arr = (_lib.BeamformArgs * 65)()
+    assert lib.mach_batch_submit_noalloc(arr, 65, None) == 1
+    assert "allocation-free maximum of 64" in _lib.last_error()
+    assert lib.mach_batch_submit_noalloc(None, 0, None) == 0
+    assert lib.mach_batch_submit_noalloc(None, 2, None) == 1
+    assert "requests is NULL" in _lib.last_error()
+    bad = (_lib.BeamformArgs * 2)(_args(n_vox=0), _args(tukey_alpha=3.0))
+    assert lib.mach_batch_submit_noalloc(bad, 2, None) == 1
+    assert _lib.last_error().startswith("request 1: Error: tukey_alpha")
+    empty = (_lib.BeamformArgs * 2)(_args(n_vox=0), _args(n_rx=0, accumulate=1, out=256, scan_coords_m=256))
+    assert lib.mach_batch_submit_noalloc(empty, 2, None) == 0, _lib.last_error()
