@@ -140,28 +140,89 @@ _ORDER_CACHE: "collections.OrderedDict" = collections.OrderedDict()
 _ORDER_CACHE_SIZE = 16
 
 
-def voxel_order(scan: torch.Tensor):
-    """Depth-major processing order of the voxels: sorted by z, then x, then y (stable).
+# Voxel tiles (depth, then the varying lateral axes) of one wave of a general-kernel layout, by voxels
+# per wave; see wave_tile.
+_PLANAR_TILES = {64: (8, 8), 32: (4, 8), 16: (4, 4)}
+_VOLUME_TILES = {64: (4, 4, 4), 32: (2, 4, 4), 16: (2, 2, 4)}
 
-    Waves that run together then beamform voxels at the same depth, whose interpolation taps hit the
-    same few channel-data rows per element, so those rows stay in L2. Any permutation gives the same
-    results (each voxel is computed independently, with a fixed summation order); the order only
-    affects speed. Cached per scan-coordinate tensor (identity and in-place version).
-    ``MACH_AMD_VOXEL_ORDER=0`` disables it (caller order).
+
+def wave_tile(layout: int):
+    """Voxels per wave of a general-kernel layout code when the wave holds 16 or more voxels (lanes per
+    voxel 1, 2 or 4), else None: lateral groups, the LDS-staged band kernel and narrow waves are tuned
+    for depth-major rows, and an automatic layout (0) is chosen by the library later."""
+    if layout <= 0 or layout & 0x8000:
+        return None
+    lf = (layout >> 4) & 0xFF
+    if lf == 0 or 64 // lf < 16:
+        return None
+    return 64 // lf
+
+
+def _depth_major(scan: torch.Tensor) -> torch.Tensor:
+    idx = torch.argsort(scan[:, 1], stable=True)
+    idx = idx[torch.argsort(scan[idx, 0], stable=True)]
+    return idx[torch.argsort(scan[idx, 2], stable=True)]
+
+
+def _tiled(scan: torch.Tensor, per_wave: int) -> torch.Tensor:
+    """Wave-sized voxel tiles. Each axis's coordinates are ranked (exact for grids); voxels are grouped
+    into tiles of (depth x lateral) ranks, `per_wave` voxels each, ordered depth-major at the tile
+    level and inside a tile. Tiles holding exactly `per_wave` voxels come first, so every wave of the
+    kernel gets one whole tile (a compact blob whose voxels share nearly the same in-aperture elements
+    and sample windows); the remaining voxels (partial tiles at the grid's edges, or a scattered cloud)
+    follow in depth-major order."""
+    n = scan.shape[0]
+    ranks = [torch.unique(scan[:, a], return_inverse=True)[1] for a in range(3)]
+    dims = [int(r.max()) + 1 for r in ranks]
+    lateral = [a for a in (0, 1) if dims[a] > 1]
+    sizes = {2: 1, 0: 1, 1: 1}
+    if len(lateral) == 2:
+        sizes[2], sizes[0], sizes[1] = _VOLUME_TILES[per_wave]
+    else:
+        sizes[2], lat = _PLANAR_TILES[per_wave]
+        for a in lateral:
+            sizes[a] = lat
+    tid = torch.zeros(n, dtype=torch.int64, device=scan.device)
+    local = torch.zeros(n, dtype=torch.int64, device=scan.device)
+    for a in (2, 0, 1):  # depth, then x, then y (y fastest)
+        nt = (dims[a] + sizes[a] - 1) // sizes[a]
+        tid = tid * nt + ranks[a] // sizes[a]
+        local = local * sizes[a] + ranks[a] % sizes[a]
+    count = torch.bincount(tid)
+    full = count[tid] == per_wave
+    dm_rank = torch.empty(n, dtype=torch.int64, device=scan.device)
+    dm_rank[_depth_major(scan)] = torch.arange(n, device=scan.device)
+    base = (int(tid.max()) + 1) * per_wave
+    key = torch.where(full, tid * per_wave + local, base + dm_rank)
+    return torch.argsort(key, stable=True)
+
+
+def voxel_order(scan: torch.Tensor, layout: int = 0):
+    """Processing order of the voxels for a launch with this layout code.
+
+    Depth-major (sorted by z, then x, then y; stable) by default: waves that run together beamform
+    voxels at the same depth, whose interpolation taps hit the same few channel-data rows per element,
+    so those rows stay in L2. For general-kernel layouts whose waves hold 16 or more voxels
+    (``wave_tile``), wave-sized tiles instead (``_tiled``): the elements a wave visits are the union of
+    its voxels' apertures, so a compact tile wastes fewer lanes than a row, and its sample windows
+    overlap more. Any permutation gives the same results (each voxel is computed independently, with a
+    fixed summation order); the order only affects speed. Cached per scan-coordinate tensor (identity
+    and in-place version) and tile. ``MACH_AMD_VOXEL_ORDER=0`` disables it (caller order),
+    ``MACH_AMD_VOXEL_ORDER=rows`` keeps depth-major rows for every layout.
     """
     n = scan.shape[0]
-    if n < _ORDER_MIN_VOXELS or not scan.is_cuda or os.environ.get("MACH_AMD_VOXEL_ORDER", "1") == "0":
+    mode = os.environ.get("MACH_AMD_VOXEL_ORDER", "1")
+    if n < _ORDER_MIN_VOXELS or not scan.is_cuda or mode == "0":
         return None
-    key = (id(scan), scan.data_ptr(), n)
+    per_wave = None if mode == "rows" else wave_tile(layout)
+    key = (id(scan), scan.data_ptr(), n, per_wave)
     hit = _ORDER_CACHE.get(key)
     if hit is not None:
         ref, version, order = hit
         if ref() is scan and version == scan._version:
             _ORDER_CACHE.move_to_end(key)
             return order
-    idx = torch.argsort(scan[:, 1], stable=True)
-    idx = idx[torch.argsort(scan[idx, 0], stable=True)]
-    idx = idx[torch.argsort(scan[idx, 2], stable=True)]
+    idx = _depth_major(scan) if per_wave is None else _tiled(scan, per_wave)
     order = idx.to(torch.int32).contiguous()
     _ORDER_CACHE[key] = (weakref.ref(scan), scan._version, order)
     while len(_ORDER_CACHE) > _ORDER_CACHE_SIZE:
@@ -228,13 +289,13 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
         raise RuntimeError("All GPU arrays must be on the same device")
     host_out = not out.is_cuda
     dev_out = torch.zeros(out.shape, dtype=out.dtype, device=device) if host_out else out
-    order = voxel_order(on_dev[2])
     if layout == 0:
         from . import autotune
 
         core = _core_shape(on_dev[0], kind)
         layout = autotune.layout_for(kind, int(interp_type), n_tx, core[-3], dev_out.shape[1], on_dev[2].shape[0],
                                      on_dev[0].data_ptr(), dev_out.data_ptr())
+    order = voxel_order(on_dev[2], layout)
     args = make_args(*on_dev, dev_out, n_tx=n_tx, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
                      sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
                      modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,

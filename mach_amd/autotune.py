@@ -204,10 +204,10 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = T
         out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.float32, device=cd.device)
     else:
         out = torch.empty((acq.n_vox, acq.n_frames), dtype=torch.complex64, device=cd.device)
-    order = _hip_impl.voxel_order(acq.scan_coords_m)
     lib = _lib.load()
 
     def run(layout):
+        order = _hip_impl.voxel_order(acq.scan_coords_m, layout)  # each candidate with its own order
         args = _hip_impl.make_args(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
                                    n_tx=acq.n_tx, kind=kind, f_number=p["f_number"], rx_start_s=p["rx_start_s"],
                                    sampling_freq_hz=p["sampling_freq_hz"], sound_speed_m_s=p["sound_speed_m_s"],

@@ -258,6 +258,7 @@ def _to_request(req: ServeRequest, data: torch.Tensor, out: torch.Tensor) -> _ba
                        voxel_order=g.voxel_order)
     r.layout = autotune.layout_for(r.kind(), r.interp_type, r.n_tx(), g.rx_coords_m.shape[0], out.shape[1],
                                    g.scan_coords_m.shape[0], data.data_ptr(), out.data_ptr())
+    r.voxel_order = voxel_order(g.scan_coords_m, r.layout)  # cached per geometry and tile
     return r
 
 

@@ -81,3 +81,30 @@ def test_analytic_signal_of_a_cosine_is_the_complex_exponential():
     t = np.arange(n)
     rf = np.cos(2 * np.pi * k * t / n)[:, None, None]
     np.testing.assert_allclose(analytic_signal64(rf)[0, 0, :, 0], np.exp(1j * 2 * np.pi * k * t / n), atol=1e-12)
+
+
+def test_bmode_db_diff_closed_forms():
+    """bmode_db_diff (the FP16 B-mode dB figure): each case fails a plausible mistake. Exact decades
+    give exact dB; a factor 2 is 20 log10 2 = 6.0206 dB (10 log10 would give 3.01); each image is
+    normalised by its own maximum (a global scale changes nothing); the -60 dB mask is taken on the
+    reference image and is strict (a voxel at exactly -60 dB, or below it, is ignored however wrong)."""
+    from oracle.das64 import bmode_db_diff
+
+    ref = np.array([1.0, 1e-1, 1e-2, 1e-3, 1e-4])
+    assert bmode_db_diff(ref, ref) == 0.0
+    assert bmode_db_diff(7.5 * ref, ref) == pytest.approx(0.0, abs=1e-12)  # own-maximum normalisation
+    y = ref.copy()
+    y[1] = 2e-1
+    assert bmode_db_diff(y, ref) == pytest.approx(20 * np.log10(2.0), rel=1e-12)
+    y = ref.copy()
+    y[3:] = [0.5, 0.9]  # only voxels at or below -60 dB in the reference differ
+    assert bmode_db_diff(y, ref) == pytest.approx(0.0, abs=1e-12)
+    # the mask comes from the reference: a voxel that collapses in y still counts
+    assert bmode_db_diff(np.array([1.0, 1e-4]), np.array([1.0, 0.5])) == pytest.approx(80 - 20 * np.log10(2.0),
+                                                                                      rel=1e-12)
+    # magnitudes of complex values; a phase change alone is no difference
+    assert bmode_db_diff(np.array([1j, -0.1]), np.array([1.0, 0.1 + 0j])) == pytest.approx(0.0, abs=1e-12)
+    # normalised by each image's own maximum: [2, 1] vs [1, 1] differ by 6.02 dB at the second voxel
+    assert bmode_db_diff(np.array([2.0, 1.0]), np.array([1.0, 1.0])) == pytest.approx(20 * np.log10(2.0), rel=1e-12)
+    # a floor other than -60 dB moves the mask
+    assert bmode_db_diff(y, ref, floor_db=-90.0) > 50
