@@ -27,7 +27,7 @@ DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning
 
 
 # (lanes_per_voxel, frames_per_lane) layouts also compiled with 2 or 4 frame groups per lane
-MULTI_GROUP = {(16, 4), (32, 2), (64, 1), (8, 4)}
+MULTI_GROUP = {(16, 4), (32, 2), (64, 1), (8, 4), (1, 4)}
 
 
 # lateral groups: 64 lanes, every lane holding its frames of 2 or 4 voxels that share sample windows
@@ -240,7 +240,7 @@ def _main():
     dev = torch.device("cuda", 0)
     for name in args.workloads.split(","):
         acq = phantoms.make(name, device=dev)
-        storages = ("fp16",) if name == "volumetric" else ("c64", "fp16")
+        storages = ("c64", "fp16")
         outputs = (0, 1) if name in args.power.split(",") else (0,)
         for st in storages:
             for output in outputs:

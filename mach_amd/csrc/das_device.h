@@ -546,23 +546,39 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
 
         for (int64_t t0 = 0; t0 < p.n_tx; t0 += TC) {
             const int nt = (int)((p.n_tx - t0) < TC ? (p.n_tx - t0) : TC);
-            double Tv[TC];
+            // Transmit arrivals are float32 values (array entries, or wave_arrival's rounding), so they
+            // are held as float32 and widened exactly where they are used: half the registers.
+            float Tv[TC];
 #pragma unroll
             for (int tt = 0; tt < TC; ++tt) {
-                Tv[tt] = 0.0;
+                Tv[tt] = 0.f;
                 if (vok && tt < nt)
-                    Tv[tt] = p.waves != nullptr ? wave_arrival(p.waves[t0 + tt], xv, yv, zv)
-                                                : (double)p.arrivals[(t0 + tt) * p.n_vox + v];
+                    Tv[tt] = p.waves != nullptr ? (float)wave_arrival(p.waves[t0 + tt], xv, yv, zv)
+                                                : p.arrivals[(t0 + tt) * p.n_vox + v];
             }
             for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
                 const int64_t e = e0 + lf;
-                PairSetup s[TC];
                 PairGeom g{0.0, 0.f};
                 if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+                // Multi-transmit chunks whose taps for the whole chunk fit 32 VGPRs (single frames, e.g.
+                // compounding: 4 x 8 dwords) load them together. Wider taps keep one transmit at a time:
+                // for the 8-frame FP16 volume (8 dwords a transmit) groups of 4 cost a wave per SIMD and
+                // measured 38.5 ms against 35.8.
+                constexpr int kTapDwords = Taps<ST, INTERP, FPL>::kN * CH * FPL * kElemBytes<ST> / 4;
+                constexpr int kTxGroup = kTapDwords * TC <= 32 ? TC : 1;
+                // One lane per voxel, one transmit at a time: the lane that computes a pair also combines
+                // it (no broadcast), so each transmit's timing (pair_time) is computed right before its
+                // combine instead of for the whole chunk up front. The chunk's PairSetup array (4 VGPRs a
+                // transmit) is never live, and the wave walks an element when some voxel has it in its
+                // aperture; out-of-record and tau < 0 pairs are skipped per transmit, as below.
+                constexpr bool kLazy = LF == 1 && TC > 1 && kTxGroup == 1;
+                PairSetup s[kLazy ? 1 : TC];
+                if constexpr (!kLazy) {
 #pragma unroll
-                for (int tt = 0; tt < TC; ++tt) {
-                    s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
-                    if (tt < nt) s[tt] = pair_time<INTERP>(p, g, Tv[tt]);
+                    for (int tt = 0; tt < TC; ++tt) {
+                        s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
+                        if (tt < nt) s[tt] = pair_time<INTERP>(p, g, (double)Tv[tt]);
+                    }
                 }
                 // Single-transmit quadratic launches: the setup lane forms the Lagrange weights once per
                 // pair and broadcasts them, instead of every lane of the voxel forming them per element
@@ -570,13 +586,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 constexpr bool kPreWeights = INTERP == kQuadratic && TC == 1;
                 // Two-element pipelining (below) for single-transmit launches whose taps of one element
                 // fit in 24 VGPRs, so that two elements' taps in flight leave room for 4+ waves per SIMD.
-                constexpr int kTapDwords = Taps<ST, INTERP, FPL>::kN * CH * FPL * kElemBytes<ST> / 4;
                 constexpr bool kPipe = TC == 1 && !WIDE && kTapDwords <= 24;
-                // Multi-transmit chunks whose taps for the whole chunk fit 32 VGPRs (single frames, e.g.
-                // compounding: 4 x 8 dwords) load them together. Wider taps keep one transmit at a time:
-                // for the 8-frame FP16 volume (8 dwords a transmit) groups of 4 cost a wave per SIMD and
-                // measured 38.5 ms against 35.8.
-                constexpr int kTxGroup = kTapDwords * TC <= 32 ? TC : 1;
                 constexpr bool kTxPipe = TC > 1 && !WIDE && kTxGroup > 1;
                 QuadWeights pw_own{0.f, 0.f, 0.f};
                 if constexpr (kPreWeights) pw_own = quad_weights(s[0].frac);
@@ -586,8 +596,12 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 // predicate is the pair's validity only: the lane that computed a pair may itself own
                 // no frames.
                 uint64_t jm = 0ull;
+                if constexpr (kLazy) {
+                    jm = __ballot(g.w != 0.f);
+                } else {
 #pragma unroll
-                for (int tt = 0; tt < TC; ++tt) jm |= __ballot(s[tt].i0 >= 0);
+                    for (int tt = 0; tt < TC; ++tt) jm |= __ballot(s[tt].i0 >= 0);
+                }
                 if constexpr (LF < 64) {
 #pragma unroll
                     for (int sh = LF; sh < 64; sh <<= 1) jm |= jm >> sh;
@@ -705,7 +719,13 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
 #pragma unroll
                         for (int tt = 0; tt < TC; ++tt) {
                             if (tt >= nt) break;
-                            const ElemCoef cf = elem_coef<LF, INTERP, kPreWeights>(s[tt], pw_own, j, lane);
+                            ElemCoef cf;
+                            if constexpr (kLazy) {
+                                const PairSetup st = pair_time<INTERP>(p, g, (double)Tv[tt]);
+                                cf = elem_weights<LF, INTERP, false>(st, pw_own, j, lane, st.i0);
+                            } else {
+                                cf = elem_coef<LF, INTERP, kPreWeights>(s[tt], pw_own, j, lane);
+                            }
                             if (cf.i0 < 0 || !fok[0]) continue;
                             // first element of the channel's [sample][frame] block (wave-uniform)
                             const int64_t row0 = ((t0 + tt) * p.n_rx + e_abs) * p.n_samples * p.n_frames;

@@ -69,9 +69,11 @@ struct DasLayout {
     int voxels_per_lane = 1;  // VPL in {1, 2, 4}; VPL > 1 only with LF = 64, FPL in {2, 4}, CH = 1
 };
 
-// (LF, FPL) layouts compiled with CH > 1 (frame ensembles, where the per-pair setup is worth sharing).
+// (LF, FPL) layouts compiled with CH > 1 (frame ensembles, where the per-pair setup is worth sharing;
+// 1 x 4 x 2 gives one lane all 8 frames of a voxel of the volumetric set, with one setup per pair).
 constexpr bool multi_group_layout(int lf, int fpl) {
-    return (lf == 16 && fpl == 4) || (lf == 32 && fpl == 2) || (lf == 64 && fpl == 1) || (lf == 8 && fpl == 4);
+    return (lf == 16 && fpl == 4) || (lf == 32 && fpl == 2) || (lf == 64 && fpl == 1) || (lf == 8 && fpl == 4) ||
+           (lf == 1 && fpl == 4);
 }
 
 // Layout code of the C ABI: (log2(CH) << 12) | (LF << 4) | FPL; LF == 0 selects the band kernel.
