@@ -230,6 +230,15 @@ def voxel_order(scan: torch.Tensor, layout: int = 0):
     return order
 
 
+def order_on(order, stream):
+    """``order`` (from voxel_order, or None) marked as used by work on ``stream``: the order cache is
+    bounded and may drop a tensor that a launch on another stream than its allocation stream still reads,
+    and the allocator must not hand that memory out again before the launch has finished."""
+    if order is not None:
+        order.record_stream(stream)
+    return order
+
+
 def _device_for(tensors):
     for t in tensors:
         if t.is_cuda:
@@ -301,7 +310,9 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
                      modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,
                      accumulate=accumulate_device_out or host_out, layout=layout, order=order)
     with torch.cuda.device(device):
-        stream = torch.cuda.current_stream(device).cuda_stream
+        cur = torch.cuda.current_stream(device)
+        order_on(order, cur)
+        stream = cur.cuda_stream
         status = _LIB.mach_beamform(ctypes.byref(args), ctypes.c_void_p(stream))
         _lib.check(status)
         if host_out:

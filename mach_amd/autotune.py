@@ -63,6 +63,17 @@ def _bucket(n: int) -> int:
     return 1 << max(0, math.ceil(math.log2(max(1, n))))
 
 
+def _parse_key(key: str):
+    """Fields of a shape_key string: frames f, voxel bucket v and the rest (storage, interpolation,
+    transmit and element buckets, output kind) as a tuple; None for a malformed key."""
+    try:
+        parts = {p[0]: p[1:] for p in key.split(":")}
+        rest = (parts["d"], parts["i"], parts["t"], parts["e"], parts.get("o", "0"))
+        return {"f": int(parts["f"]), "v": int(parts["v"]), "rest": rest}
+    except (KeyError, ValueError, IndexError):
+        return None
+
+
 def shape_key(kind: int, interp: int, n_tx: int, n_rx: int, n_frames: int, n_vox: int, output: int = 0) -> str:
     """Shape class of a request; ``output`` (mach_output_kind) other than the signal gets its own entry."""
     key = f"d{kind}:i{int(interp)}:t{_bucket(n_tx)}:e{_bucket(n_rx)}:f{n_frames}:v{_bucket(n_vox)}"
@@ -143,7 +154,30 @@ class TuningTable:
             self.entries = {k: int(v["layout"]) for k, v in data.get("entries", {}).items()}
 
     def lookup(self, key: str) -> int:
-        return self.entries.get(key, 0)
+        """The layout of shape class ``key``; without an entry, that of the nearest tuned class
+        (``nearest``), or 0 (the library's heuristic)."""
+        hit = self.entries.get(key)
+        return hit if hit is not None else self.nearest(key)
+
+    def nearest(self, key: str, max_octaves: int = 3) -> int:
+        """Layout of the tuned class closest to ``key``: the same storage, interpolation, transmit and
+        element octaves, frame count and output kind, at the nearest voxel octave at most
+        ``max_octaves`` away (ties to the larger): a grid scaled from a tuned one. 0 when there is none."""
+        want = _parse_key(key)
+        if want is None:
+            return 0
+        best, best_rank = 0, None
+        for k, lay in self.entries.items():
+            have = _parse_key(k)
+            if have is None or have["rest"] != want["rest"] or have["f"] != want["f"]:
+                continue
+            dv = abs(have["v"].bit_length() - want["v"].bit_length())
+            if dv > max_octaves:
+                continue
+            rank = (dv, -have["v"])
+            if best_rank is None or rank < best_rank:
+                best, best_rank = lay, rank
+        return best
 
     def record(self, key: str, layout: int, times: dict | None = None, meta: dict | None = None) -> None:
         with self._lock:
@@ -236,20 +270,27 @@ def _main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--power", default="doppler,pymust_disk", help="workloads also tuned for the power Doppler output")
+    ap.add_argument("--volumetric-frames", default="16,32",
+                    help="extra frame counts of the FP16 volume: the ensembles a server fuses 8-frame volumes into")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     for name in args.workloads.split(","):
-        acq = phantoms.make(name, device=dev)
-        storages = ("c64", "fp16")
-        outputs = (0, 1) if name in args.power.split(",") else (0,)
-        for st in storages:
-            for output in outputs:
-                key, best, times = tune_acquisition(acq, st, args.reps, output=output)
-                ranked = sorted(times.items(), key=lambda kv: kv[1])
-                print(f"{name:11s} {st:5s} {key}: best {hex(best)} {times[best]:.4f} ms; "
-                      + ", ".join(f"{hex(k)}={v:.4f}" for k, v in ranked[:6]), flush=True)
-        del acq
-        torch.cuda.empty_cache()
+        frame_counts = [None]
+        if name == "volumetric" and args.volumetric_frames:
+            frame_counts += [int(f) for f in args.volumetric_frames.split(",")]
+        for nf in frame_counts:
+            acq = phantoms.make(name, device=dev) if nf is None else phantoms.make(name, device=dev, n_frames=nf)
+            storages = ("c64", "fp16") if nf is None else ("fp16",)
+            outputs = (0, 1) if name in args.power.split(",") else (0,)
+            reps = args.reps if nf is None else max(3, args.reps * 8 // nf)  # long launches: fewer repetitions
+            for st in storages:
+                for output in outputs:
+                    key, best, times = tune_acquisition(acq, st, reps, output=output)
+                    ranked = sorted(times.items(), key=lambda kv: kv[1])
+                    print(f"{name:11s} {st:5s} {key}: best {hex(best)} {times[best]:.4f} ms; "
+                          + ", ".join(f"{hex(k)}={v:.4f}" for k, v in ranked[:6]), flush=True)
+            del acq
+            torch.cuda.empty_cache()
     print("saved", table().save(args.out), flush=True)
 
 

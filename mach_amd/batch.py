@@ -82,8 +82,9 @@ class Request:
 def run_plain(req: Request, stream=None) -> None:
     """One ``mach_beamform`` launch for ``req`` (the per-request baseline of the batched path)."""
     a = req.args()
-    s = torch.cuda.current_stream().cuda_stream if stream is None else stream.cuda_stream
-    _lib.check(_hip_impl._LIB.mach_beamform(ctypes.byref(a), ctypes.c_void_p(s)))
+    st = torch.cuda.current_stream() if stream is None else stream
+    _hip_impl.order_on(req.voxel_order, st)
+    _lib.check(_hip_impl._LIB.mach_beamform(ctypes.byref(a), ctypes.c_void_p(st.cuda_stream)))
 
 
 def submit_noalloc(requests, stream=None) -> None:
@@ -94,8 +95,11 @@ def submit_noalloc(requests, stream=None) -> None:
     arr = (_lib.BeamformArgs * max(n, 1))()
     for i, r in enumerate(requests):
         arr[i] = r.args()
-    s = torch.cuda.current_stream().cuda_stream if stream is None else stream.cuda_stream
-    _lib.check(_hip_impl._LIB.mach_batch_submit_noalloc(arr, n, ctypes.c_void_p(s)), "mach_batch_submit_noalloc")
+    st = torch.cuda.current_stream() if stream is None else stream
+    for r in requests:
+        _hip_impl.order_on(r.voxel_order, st)
+    _lib.check(_hip_impl._LIB.mach_batch_submit_noalloc(arr, n, ctypes.c_void_p(st.cuda_stream)),
+               "mach_batch_submit_noalloc")
 
 
 class BatchRunner:
@@ -116,8 +120,11 @@ class BatchRunner:
         arr = (_lib.BeamformArgs * max(n, 1))()
         for i, r in enumerate(requests):
             arr[i] = r.args()
-        s = torch.cuda.current_stream().cuda_stream if stream is None else stream.cuda_stream
-        _lib.check(_hip_impl._LIB.mach_batch_submit(self._h, arr, n, ctypes.c_void_p(s)), "mach_batch_submit")
+        st = torch.cuda.current_stream() if stream is None else stream
+        for r in requests:
+            _hip_impl.order_on(r.voxel_order, st)
+        _lib.check(_hip_impl._LIB.mach_batch_submit(self._h, arr, n, ctypes.c_void_p(st.cuda_stream)),
+                   "mach_batch_submit")
 
     def close(self) -> None:
         if self._h:

@@ -27,7 +27,7 @@ import torch
 
 from . import _lib
 from ._hip_impl import (OUTPUT_ENVELOPE, OUTPUT_POWER, InterpolationType, _as_tensor, _core_shape, _data_kind,
-                        _device_for, _require, _shape_str, check_dimensions, make_args, voxel_order)
+                        _device_for, _require, _shape_str, check_dimensions, make_args, order_on, voxel_order)
 
 __all__ = ["power_doppler", "envelope", "bmode_db", "power_doppler_db", "to_db"]
 
@@ -86,7 +86,9 @@ def _launch(cd, rx, scan, arr, out, peak, *, kind, n_tx, output, accumulate, f_n
     args = make_args(cd, rx, scan, arr, out, n_tx=n_tx, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
                      sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
                      modulation_freq_hz=fmod, tukey_alpha=tukey_alpha, interp_type=interp_type,
-                     accumulate=accumulate, layout=layout, order=voxel_order(scan, layout), output=output, peak=peak)
+                     accumulate=accumulate, layout=layout,
+                     order=order_on(voxel_order(scan, layout), torch.cuda.current_stream(device)), output=output,
+                     peak=peak)
     lib = _lib.load()
     with torch.cuda.device(device):
         stream = torch.cuda.current_stream(device).cuda_stream

@@ -9,9 +9,11 @@ server:
   lands in its ring slot when it arrives, the way an acquisition front end DMAs into device memory;
 * admits arrived requests in arrival order while their channel-data and output bytes fit the HBM
   budget (the rings' capacity). Later arrivals wait until completed work frees space;
-* packs every admitted, not-yet-running request into one batch. Requests of a batch that share a
-  geometry (probe, grid, transmits, dtype) are fused into one ensemble along the frame axis, so their
-  float64 pair geometry is computed once. Small requests (and fused groups) go into one
+* packs every admitted, not-yet-running request into one batch. Single-transmit requests of a batch
+  that share a geometry (probe, grid, dtype) are fused into one ensemble along the frame axis, so their
+  float64 pair geometry is computed once (``fusable``: multi-transmit requests gain nothing from it). The ensemble's concatenated channel data and output are
+  reserved in the rings too (the members run separately when there is no room), so HBM in use never
+  exceeds the budget. Small requests (and fused groups) go into one
   persistent-kernel submission (``BatchRunner``, graph replay); a request or group big enough to fill
   the GPU by itself (``plain_min_voxel_frames``) is launched as its own plain kernel on the batch's
   stream, where it gets the launch-shape choices (frame groups, balanced XCD remap) a work queue
@@ -21,8 +23,10 @@ server:
 * while the GPU is busy, holds large requests (``hold_min_points``) back for up to ``max_hold_s`` so
   that same-geometry requests arriving meanwhile fuse with them; requests therefore complete out of
   admission order, and the rings release slots in any order;
+* isolates failures: a request that fails validation or launch completes as failed (``error`` set,
+  no output) while the others of its batch run, and the stream goes on;
 * reports per-request latency (arrival -> result ready on the device), p50/p99, and throughput in
-  points/s over the makespan.
+  points/s over the makespan (completed requests only).
 
 ``serve_plain`` runs the same arrival stream one request at a time through the plain kernel (one
 launch and one synchronisation per request): the baseline the batched server is compared with.
@@ -42,6 +46,7 @@ import torch
 from . import autotune
 from . import batch as _batch
 from . import phantoms
+from . import _hip_impl
 from ._hip_impl import voxel_order
 
 GIB = 1 << 30
@@ -54,6 +59,16 @@ class Geometry:
     tx_wave_arrivals_s: torch.Tensor
     voxel_order: torch.Tensor | None
     params: dict
+    orders: dict = dataclasses.field(default_factory=dict)  # voxel order per wave tile, held for the geometry
+
+    def order_for(self, layout: int):
+        """The voxel processing order for a launch with this layout code, computed once per wave tile and
+        held by the geometry: batches in flight on other streams read it, so it must outlive them (the
+        library's own order cache is bounded and may drop it)."""
+        per_wave = _hip_impl.wave_tile(layout)
+        if per_wave not in self.orders:
+            self.orders[per_wave] = voxel_order(self.scan_coords_m, layout)
+        return self.orders[per_wave]
 
 
 @dataclasses.dataclass
@@ -70,6 +85,7 @@ class ServeRequest:
     out: torch.Tensor | None = None
     data_slot: tuple | None = None
     out_slot: tuple | None = None
+    error: str | None = None  # why the request failed (it then has no output)
 
     @property
     def data_bytes(self) -> int:
@@ -258,7 +274,7 @@ def _to_request(req: ServeRequest, data: torch.Tensor, out: torch.Tensor) -> _ba
                        voxel_order=g.voxel_order)
     r.layout = autotune.layout_for(r.kind(), r.interp_type, r.n_tx(), g.rx_coords_m.shape[0], out.shape[1],
                                    g.scan_coords_m.shape[0], data.data_ptr(), out.data_ptr())
-    r.voxel_order = voxel_order(g.scan_coords_m, r.layout)  # cached per geometry and tile
+    r.voxel_order = g.order_for(r.layout)
     return r
 
 
@@ -275,6 +291,7 @@ class ServeReport:
     n_batches: int = 0
     n_fused: int = 0  # requests that ran fused with others of the same geometry
     n_direct: int = 0  # requests (or fused groups' members) launched as their own plain kernel in a batch
+    n_failed: int = 0  # requests that failed (validation or launch); not counted in points or latency
 
     @property
     def gpts_per_s(self) -> float:
@@ -287,20 +304,66 @@ class ServeReport:
 
 
 def _report(mode, reqs, t_first, max_batch=1, n_batches=0) -> ServeReport:
-    lat = np.array([r.latency_s for r in reqs]) * 1e3
+    """Throughput over the points of the requests that completed; latencies of those requests."""
+    ok = [r for r in reqs if r.error is None]
+    lat = np.array([r.latency_s for r in ok]) * 1e3 if ok else np.array([np.nan])
     end = max(r.done_s for r in reqs)
-    return ServeReport(mode, len(reqs), end - t_first, sum(r.points for r in reqs),
-                       float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), float(lat.mean()),
-                       max_batch, n_batches)
+    rep = ServeReport(mode, len(reqs), end - t_first, sum(r.points for r in ok),
+                      float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), float(lat.mean()),
+                      max_batch, n_batches)
+    rep.n_failed = len(reqs) - len(ok)
+    return rep
+
+
+def _submit_isolated(submit_batch, run_direct, batched, direct, after_batched=None):
+    """Launches one batch's work so that a request that fails does not take the others with it.
+
+    ``batched`` and ``direct`` are lists of (request, members): ``submit_batch`` takes a list of
+    requests as one persistent-kernel submission, ``run_direct`` one request as its own launch. A batch
+    submission validates every request before it enqueues anything (mach_batch_submit is all or
+    nothing), so when it is refused each request is submitted alone and only those refused again fail.
+    ``after_batched`` (if given) is called between the batched and the direct launches. Returns
+    [(members, message)] of the failed requests."""
+    failed = []
+    if batched:
+        try:
+            submit_batch([r for r, _ in batched])
+        except (RuntimeError, ValueError):
+            for r, members in batched:
+                try:
+                    submit_batch([r])
+                except (RuntimeError, ValueError) as e:
+                    failed.append((members, str(e)))
+    if after_batched is not None:
+        after_batched()
+    for r, members in direct:
+        try:
+            run_direct(r)
+        except (RuntimeError, ValueError) as e:
+            failed.append((members, str(e)))
+    return failed
+
+
+def fusable(r) -> bool:
+    """Whether fusing the request with same-geometry partners pays: single-transmit ensembles, whose
+    wider frame groups share one float64 setup per (voxel, element) across more lanes. Multi-transmit
+    requests (compounding, the volume) run alone: their tuned layouts give a lane all of a voxel's
+    frames, so a fused ensemble only adds frame blocks that each repeat the setup (measured on the
+    8-frame FP16 volume: 23.8 ms alone, 24.8 ms per 8 frames fused to 16, 29.7 ms fused to 32)."""
+    return r.source.shape[0] == 1
 
 
 def _fusion_groups(reqs, fuse=True, max_frames=256):
-    """Groups of requests that can run as one ensemble: same geometry object (probe, grid, transmits,
-    parameters), storage dtype and [n_tx, n_rx, n_samples]; frames concatenated up to ``max_frames``."""
+    """Groups of requests that can run as one ensemble: fusable requests (``fusable``) with the same
+    geometry object (probe, grid, transmits, parameters), storage dtype and [n_tx, n_rx, n_samples];
+    frames concatenated up to ``max_frames``."""
     if not fuse:
         return [[r] for r in reqs]
     groups, open_ = [], {}
     for r in reqs:
+        if not fusable(r):
+            groups.append([r])
+            continue
         key = (id(r.geometry), r.source.dtype, tuple(r.source.shape[:3]), tuple(r.source.shape[4:]))
         g = open_.get(key)
         nf = r.source.shape[3]
@@ -323,7 +386,7 @@ def _select(admitted, gpu_busy, now, hold_min_points, max_hold_s, max_fused_fram
     groups = collections.OrderedDict()
     for item in admitted:
         r = item[0]
-        if r.points >= hold_min_points:
+        if r.points >= hold_min_points and fusable(r):
             groups.setdefault((id(r.geometry), r.source.dtype, tuple(r.source.shape[:3])), []).append(item)
     if not groups:
         return admitted, []
@@ -333,6 +396,48 @@ def _select(admitted, gpu_busy, now, hold_min_points, max_hold_s, max_fused_fram
                if k == largest or frames[k] >= max_fused_frames or now - min(it[0].admit_s for it in g) >= max_hold_s}
     held = {id(it) for k, g in groups.items() if k not in go_keys for it in g}
     return [it for it in admitted if id(it) not in held], [it for it in admitted if id(it) in held]
+
+
+def _reserve_fusion(group, data_ring, out_ring, extra):
+    """Splits a fusion group into the groups that run: the whole group as one ensemble over its
+    concatenated frames when the rings have room for that ensemble's channel data and output (their
+    slots are appended to ``extra`` and released with the batch, so the HBM budget covers them), else
+    each member on its own. Returns [(members, (data_slot, out_slot) or None)]."""
+    if len(group) == 1:
+        return [(group, None)]
+    n_frames = sum(r.source.shape[3] for r in group)
+    n_vox = group[0].geometry.scan_coords_m.shape[0]
+    data_bytes, out_bytes = sum(r.data_bytes for r in group), n_vox * n_frames * 8
+    ds = os_ = None
+    if data_bytes <= data_ring.capacity and out_bytes <= out_ring.capacity:  # else it can never fit
+        ds = data_ring.alloc(data_bytes)
+        os_ = out_ring.alloc(out_bytes) if ds is not None else None
+        if ds is not None and os_ is None:
+            data_ring.undo_last()
+    if ds is None or os_ is None:
+        return [([r], None) for r in group]
+    extra += [(data_ring, ds), (out_ring, os_)]
+    return [(group, (ds, os_))]
+
+
+def _prepare(members, slots, data_ring, out_ring, stream):
+    """(request, fused channel data or None) of a group from _reserve_fusion; sets each member's out."""
+    if slots is None:
+        r = members[0]
+        r.out = _out_view(r, out_ring, r.out_slot)
+        return _to_request(r, _data_view(r, data_ring, r.data_slot), r.out), None
+    shape = list(members[0].source.shape)
+    shape[3] = sum(r.source.shape[3] for r in members)
+    fused = data_ring.view(slots[0], members[0].source.dtype, tuple(shape))
+    out_f = out_ring.view(slots[1], torch.complex64, (members[0].geometry.scan_coords_m.shape[0], shape[3]))
+    with torch.cuda.stream(stream):
+        torch.cat([_data_view(r, data_ring, r.data_slot) for r in members], dim=3, out=fused)
+    a = 0
+    for r in members:
+        nf = r.source.shape[3]
+        r.out = out_f[:, a:a + nf]
+        a += nf
+    return _to_request(members[0], fused, out_f), fused
 
 
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
@@ -354,7 +459,11 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival_s))
     waiting = collections.deque()
     admitted = []  # (req, copy_event)
-    inflight = collections.deque()  # (reqs, event)
+    # completion marks of launched work: [batch, members, event, kept objects, extra ring slots]. A batch
+    # leaves one mark after its persistent-kernel submission, one per direct launch and a final one
+    # (failed members, fused buffers, references); requests complete at their own mark.
+    marks = []
+    live_batches = collections.Counter()  # batch -> marks not yet complete
     n_done, n_batches, biggest, n_fused, n_direct = 0, 0, 0, 0, 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -379,22 +488,32 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
                 ev = torch.cuda.Event()
                 ev.record(copy_stream)
             admitted.append((r, ev))
-        # completions (batches finish in order)
-        while inflight and inflight[0][1].query():
-            done_reqs, _, _ = inflight.popleft()
+        # completions: every mark whose work has finished
+        still = []
+        for mk in marks:
+            batch_id, done_reqs, ev, _, extra = mk
+            if not ev.query():
+                still.append(mk)
+                continue
             t_done = time.perf_counter() - t0
             for r in done_reqs:
                 r.done_s = t_done
-                if keep_outputs:  # private copy, complete before the ring slot can be reused
+                if keep_outputs and r.out is not None:  # private copy, complete before its slot is reused
                     r.out = r.out.clone()
                     torch.cuda.current_stream(device).synchronize()
                 data_ring.free(r.data_slot)
                 out_ring.free(r.out_slot)
+            for ring, slot in extra:  # fused groups' data and output
+                ring.free(slot)
             n_done += len(done_reqs)
+            live_batches[batch_id] -= 1
+            if live_batches[batch_id] == 0:
+                del live_batches[batch_id]
+        marks = still
         # launch: what is admitted (less the large requests held for fusion partners) forms a batch
         take = []
-        if admitted and len(inflight) < 2:
-            go, admitted = _select(admitted, len(inflight) > 0, time.perf_counter() - t0, hold_min_points,
+        if admitted and len(live_batches) < 2:
+            go, admitted = _select(admitted, len(live_batches) > 0, time.perf_counter() - t0, hold_min_points,
                                    max_hold_s, max_fused_frames)
             take = go[:max_batch]
             admitted = go[max_batch:] + admitted
@@ -402,48 +521,70 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             stream = compute[n_batches % 2]
             for _r, ev in take:
                 stream.wait_event(ev)
-            breqs, direct, keep = [], [], []
-            for group in _fusion_groups([r for r, _ in take], fuse, max_fused_frames):
-                if len(group) == 1:
-                    r = group[0]
-                    r.out = _out_view(r, out_ring, r.out_slot)
-                    req = _to_request(r, _data_view(r, data_ring, r.data_slot), r.out)
-                else:
-                    # same geometry: one ensemble request over the concatenated frames
-                    with torch.cuda.stream(stream):
-                        fused = torch.cat([_data_view(r, data_ring, r.data_slot) for r in group], dim=3)
-                        out_f = torch.empty((group[0].geometry.scan_coords_m.shape[0], fused.shape[3]),
-                                            dtype=torch.complex64, device=device)
-                    a = 0
-                    for r in group:
-                        nf = r.source.shape[3]
-                        r.out = out_f[:, a:a + nf]
-                        a += nf
-                    req = _to_request(group[0], fused, out_f)
-                    keep.append(fused)
-                    n_fused += len(group)
+            batched, direct, keep, extra = [], [], [], []
+            failed = []
+            groups = [sub for g in _fusion_groups([r for r, _ in take], fuse, max_fused_frames)
+                      for sub in _reserve_fusion(g, data_ring, out_ring, extra)]
+            for members, slots in groups:
+                try:
+                    req, fused = _prepare(members, slots, data_ring, out_ring, stream)
+                except (RuntimeError, ValueError) as e:
+                    failed.append((members, str(e)))
+                    continue
+                # every tensor a launch reads (channel data, output, coordinates, arrivals, voxel order)
+                # stays referenced until its batch completes: the kernels run on another stream than
+                # the one these tensors were allocated on, so a tensor freed early could be handed out
+                # again by the allocator while a kernel still reads it
+                keep.append(req)
+                if fused is not None:
+                    n_fused += len(members)
                 if plain_min_voxel_frames is not None and req.out.numel() >= plain_min_voxel_frames:
-                    direct.append(req)
-                    n_direct += len(group)
+                    direct.append((req, members))
+                    n_direct += len(members)
                 else:
-                    breqs.append(req)
-            if breqs:  # small requests first: they finish sooner
-                runner.submit(breqs, stream=stream)
-            for req in direct:  # GPU-filling requests: their own launch, tuned layout, balanced remap
+                    batched.append((req, members))
+            # small requests first: they finish sooner; GPU-filling ones get their own launch (tuned
+            # layout, balanced XCD remap). An event after the submission and after each direct launch
+            # lets those requests complete without waiting for the rest of the batch.
+            new_marks = []
+
+            def mark(members):
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                new_marks.append([n_batches, members, ev, None, []])
+
+            def run_direct(req):
                 _batch.run_plain(req, stream=stream)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-            inflight.append(([r for r, _ in take], ev, keep))
+                mark(direct_members[id(req)])
+
+            direct_members = {id(req): members for req, members in direct}
+            fails = _submit_isolated(lambda rs: runner.submit(rs, stream=stream), run_direct, batched, direct,
+                                     after_batched=lambda: mark([]))
+            failed += fails
+            failed_ids = {id(members) for members, _ in fails}
+            if batched:  # the submission's mark covers the batched requests that were not refused
+                sub = next(mk for mk in new_marks if mk[1] == [])
+                sub[1] = [r for req, members in batched if id(members) not in failed_ids for r in members]
+            else:
+                new_marks = [mk for mk in new_marks if mk[1]]
+            for members, msg in failed:
+                for r in members:
+                    r.error, r.out = msg, None
+            # the final mark: failed members, fused buffers, and every launched request's tensors
+            mark([r for members, _ in failed for r in members])
+            new_marks[-1][3], new_marks[-1][4] = keep, extra
+            marks += new_marks
+            live_batches[n_batches] = len(new_marks)
             n_batches += 1
             biggest = max(biggest, len(take))
-        elif not admitted and not inflight and pending and not waiting:
+        elif not admitted and not marks and pending and not waiting:
             # idle until the next arrival
             dt = pending[0].arrival_s - (time.perf_counter() - t0)
             if dt > 2e-4:
                 time.sleep(dt - 1e-4)
     torch.cuda.synchronize()
     runner.close()
-    rep = _report("batched", reqs, reqs[0].arrival_s if reqs else 0.0, biggest, n_batches)
+    rep = _report("batched", reqs, min(r.arrival_s for r in reqs) if reqs else 0.0, biggest, n_batches)
     rep.n_fused = n_fused
     rep.n_direct = n_direct
     del data_ring, out_ring
@@ -468,7 +609,10 @@ def serve_plain(reqs, device, keep_outputs=False):
         data.copy_(r.source, non_blocking=True)
         out = torch.empty((r.geometry.scan_coords_m.shape[0], r.source.shape[3]), dtype=torch.complex64,
                           device=device)
-        _batch.run_plain(_to_request(r, data, out), stream=stream)
+        try:
+            _batch.run_plain(_to_request(r, data, out), stream=stream)
+        except (RuntimeError, ValueError) as e:  # this request fails; the stream goes on
+            r.error, out = str(e), None
         stream.synchronize()
         r.done_s = time.perf_counter() - t0
         r.out = out if keep_outputs else None

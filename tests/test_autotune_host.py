@@ -105,3 +105,25 @@ def test_shipped_table_entries_are_candidates_of_their_shape():
         cands = autotune.candidates(n_frames, band=single and output == 0, lateral=single)
         assert int(entry["layout"]) in cands, (key, hex(int(entry["layout"])))
         assert entry["layout_hex"] == hex(int(entry["layout"]))
+
+
+def test_nearest_class_fallback():
+    """A shape class without an entry takes the tuned layout of the nearest class: same storage,
+    interpolation, transmit/element octaves, frame count and output; nearest voxel octave within three."""
+    t = autotune.TuningTable(path="/nonexistent/table.json")
+    key = autotune.shape_key
+    t.record(key(2, 1, 5, 1024, 8, 2097152), 0x1014)
+    t.record(key(2, 1, 5, 1024, 8, 65536), 0x24)
+    t.record(key(2, 1, 5, 1024, 16, 2097152), 0x2014)
+    t.record(key(2, 1, 5, 1024, 8, 262144, output=1), 0x44)
+    assert t.lookup(key(2, 1, 5, 1024, 8, 2097152)) == 0x1014          # exact
+    assert t.lookup(key(2, 1, 5, 1024, 8, 884736)) == 0x1014           # a voxel octave away
+    assert t.lookup(key(2, 1, 5, 1024, 8, 524288)) == 0x1014           # 2 octaves from 2^21, 3 from 2^16
+    assert t.lookup(key(2, 1, 5, 1024, 8, 262144)) == 0x24             # 2 octaves from 2^16, 3 from 2^21
+    t.record(key(2, 1, 5, 1024, 8, 1048576), 0x44)
+    assert t.lookup(key(2, 1, 5, 1024, 8, 262144)) == 0x44             # 2 octaves from 2^16 and 2^20: the larger
+    assert t.lookup(key(2, 1, 5, 1024, 24, 2097152)) == 0              # other frame counts stay apart
+    assert t.lookup(key(1, 1, 5, 1024, 8, 2097152)) == 0               # other storage
+    assert t.lookup(key(2, 1, 5, 1024, 8, 2097152, output=1)) == 0x44  # output kinds stay apart
+    assert t.lookup(key(2, 1, 5, 1024, 8, 1024)) == 0                  # too many octaves away
+    assert t.lookup(key(2, 1, 75, 1024, 8, 2097152)) == 0              # other transmit octave

@@ -79,3 +79,50 @@ def test_budget_backpressure_keeps_results(device, stream_reqs):
     assert rep.n_requests == len(stream_reqs)
     for r in stream_reqs:
         assert torch.equal(r.out, alone[r.rid]), r.rid
+
+
+def test_a_failing_request_does_not_stop_the_others(device, burst_reqs):
+    """Injected failure: one request of the burst carries an invalid parameter (tukey_alpha = 2, which
+    the library refuses before any launch). It completes as failed, with the library's message, in both
+    paths; every other request's result is bitwise its result alone."""
+    import dataclasses
+
+    reqs = list(burst_reqs)
+    bad = reqs[5]
+    bad_geo = dataclasses.replace(bad.geometry, params=dict(bad.geometry.params, tukey_alpha=2.0))
+    reqs[5] = dataclasses.replace(bad, geometry=bad_geo)
+    _reset(reqs)
+    plain = serving.serve_plain(reqs, device, keep_outputs=True)
+    alone = {r.rid: r.out for r in reqs}
+    assert plain.n_failed == 1 and "tukey_alpha" in reqs[5].error and reqs[5].out is None
+    for r in reqs:
+        r.error = None
+    _reset(reqs)
+    mixed = serving.serve_batched(reqs, device, budget_bytes=8 << 30, keep_outputs=True)
+    assert mixed.n_failed == 1 and mixed.n_requests == len(reqs)
+    assert "tukey_alpha" in reqs[5].error and reqs[5].out is None
+    for i, r in enumerate(reqs):
+        if i != 5:
+            assert r.error is None and torch.equal(r.out, alone[r.rid]), r.rid
+
+
+def test_voxel_orders_outlive_the_order_cache(device, monkeypatch):
+    """Batches run on the server's own streams and read each request's voxel order there; the
+    library's order cache is bounded. With the cache cut to two entries (so orders are dropped from it
+    while batches are in flight), every result must still be bitwise its result alone: the geometry
+    holds its orders and the server keeps every launched request alive until its batch completes."""
+    from mach_amd import _hip_impl
+
+    reqs = serving.make_stream(device, n_requests=40, qps=1e6, seed=21, template_scale=0.5,
+                               frames=(1, 2, 4, 8, 16, 32, 64))
+    monkeypatch.setattr(_hip_impl, "_ORDER_CACHE_SIZE", 2)
+    _hip_impl._ORDER_CACHE.clear()
+    serving.serve_plain(reqs, device, keep_outputs=True)
+    alone = {r.rid: r.out for r in reqs}
+    _reset(reqs)
+    rep = serving.serve_batched(reqs, device, budget_bytes=16 << 30, keep_outputs=True)
+    assert rep.n_requests == len(reqs) and rep.n_failed == 0
+    assert sum(1 for g in {id(r.geometry): r.geometry for r in reqs}.values() for o in g.orders.values()
+               if o is not None) > 2  # more orders in use than the cache holds
+    for r in reqs:
+        assert torch.equal(r.out, alone[r.rid]), r.rid

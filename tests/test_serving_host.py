@@ -123,3 +123,83 @@ def test_select_holds_large_requests_for_partners():
     assert [it[0].rid for it in held] == [2]
     go, held = _select(adm, gpu_busy=True, now=2.0, hold_min_points=50, max_hold_s=1.0, max_fused_frames=256)
     assert held == []  # waited long enough
+
+
+def test_failure_isolation_resubmits_a_refused_batch_request_by_request():
+    """A batch submission is all or nothing: when it is refused, each request is submitted alone and
+    only the one that fails again is reported; direct launches fail one by one. (Injected failure on
+    the host: the fake submit refuses any batch holding request 'bad'.)"""
+    from mach_amd.serving import _submit_isolated
+
+    calls = []
+
+    def submit(rs):
+        calls.append(list(rs))
+        if "bad" in rs:
+            raise RuntimeError("request 0: Error: tukey_alpha must be in range [0, 1], but got 2.000000")
+
+    ran = []
+
+    def run_direct(r):
+        if r == "bad-direct":
+            raise RuntimeError("HIP error in beamform kernel launch")
+        ran.append(r)
+
+    batched = [("a", ["A"]), ("bad", ["B1", "B2"]), ("c", ["C"])]
+    direct = [("d", ["D"]), ("bad-direct", ["E"])]
+    failed = _submit_isolated(submit, run_direct, batched, direct)
+    assert calls[0] == ["a", "bad", "c"]  # the whole batch first
+    assert calls[1:] == [["a"], ["bad"], ["c"]]  # then one by one
+    assert [m for m, _ in failed] == [["B1", "B2"], ["E"]]
+    assert "tukey_alpha" in failed[0][1] and "HIP error" in failed[1][1]
+    assert ran == ["d"]
+    # a batch that is accepted is submitted once
+    calls.clear()
+    assert _submit_isolated(submit, run_direct, [("a", ["A"]), ("c", ["C"])], []) == []
+    assert calls == [["a", "c"]]
+
+
+def test_report_counts_only_completed_requests():
+    import torch
+
+    from mach_amd.serving import ServeRequest, _report
+
+    reqs = []
+    for i, err in enumerate([None, "bad", None]):
+        r = ServeRequest(i, 0.1 * i, None, torch.zeros(1), "plane_wave", 100 * (i + 1))
+        r.done_s, r.error = 0.1 * i + 0.05 * (i + 1), err
+        reqs.append(r)
+    rep = _report("batched", reqs, 0.0)
+    assert rep.n_failed == 1 and rep.n_requests == 3
+    assert rep.points == 100 + 300
+    assert abs(rep.p50_ms - 100.0) < 1e-9  # latencies 50 and 150 ms (the failed one is left out)
+
+
+def test_fused_groups_reserve_their_buffers_in_the_rings():
+    """A fusion group's concatenated channel data and output take ring space of their own (so the HBM
+    budget covers them); without room, the members run separately and nothing stays reserved."""
+    import types
+
+    import torch
+
+    from mach_amd.serving import DeviceRing, ServeRequest, _reserve_fusion
+
+    geo = types.SimpleNamespace(scan_coords_m=torch.zeros((100, 3)))
+    group = [ServeRequest(i, 0.0, geo, torch.zeros((1, 4, 8, f), dtype=torch.complex64), "x", 0)
+             for i, f in enumerate((3, 5))]
+    data_bytes = sum(r.data_bytes for r in group)  # 1*4*8*8 frames * 8 B
+    data_ring, out_ring = DeviceRing(1 << 16, "cpu"), DeviceRing(1 << 16, "cpu")
+    extra = []
+    runs = _reserve_fusion(group, data_ring, out_ring, extra)
+    assert len(runs) == 1 and runs[0][0] == group and runs[0][1] is not None
+    assert [(ring is data_ring, slot[1]) for ring, slot in extra] == [(True, (data_bytes + 255) // 256 * 256),
+                                                                      (False, (100 * 8 * 8 + 255) // 256 * 256)]
+    assert data_ring.used >= data_bytes and out_ring.used >= 100 * 8 * 8
+    # no room for the fused output: both members run alone, nothing is left reserved
+    tiny_out = DeviceRing(256, "cpu")
+    extra2 = []
+    data2 = DeviceRing(1 << 16, "cpu")
+    runs = _reserve_fusion(group, data2, tiny_out, extra2)
+    assert [m for m, slots in runs] == [[group[0]], [group[1]]] and all(s is None for _, s in runs)
+    assert extra2 == [] and data2.used == 0 and tiny_out.used == 0
+    assert _reserve_fusion(group[:1], data2, tiny_out, extra2) == [(group[:1], None)]
