@@ -75,7 +75,10 @@ typedef enum mach_status {
     MACH_ERR_DIMENSION = 2,        /* reference: "Dimension mismatch ..." (kernel.cu:914-951) */
     MACH_ERR_HIP = 3,              /* reference: checkCudaErrors -> exit(1); here a status    */
     MACH_ERR_NO_DEVICE = 4,
-    MACH_ERR_UNSUPPORTED = 5
+    MACH_ERR_UNSUPPORTED = 5,
+    MACH_ERR_DEBUG_CHECK = 6       /* debug build (libmach_hip_debug.so) only: a kernel bounds check
+                                      failed (the reference's CUDA_DEBUG asserts, kernel.cu:85-92);
+                                      the access was skipped, the result is not valid               */
 } mach_status;
 
 /* General beamform request. Replaces, in one call, the reference's per-transmit loop

@@ -11,7 +11,12 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "_lib")
-LIB_PATH = os.path.join(LIB_DIR, "libmach_hip.so")
+RELEASE_PATH = os.path.join(LIB_DIR, "libmach_hip.so")
+DEBUG_PATH = os.path.join(LIB_DIR, "libmach_hip_debug.so")
+# MACH_AMD_DEBUG=1 loads the debug build: the kernels count failed bounds checks (instead of trapping)
+# and every call waits for its launches and fails with MACH_ERR_DEBUG_CHECK if any check failed.
+DEBUG = os.environ.get("MACH_AMD_DEBUG", "0") not in ("", "0")
+LIB_PATH = DEBUG_PATH if DEBUG else RELEASE_PATH
 
 MACH_OK = 0
 STATUS_NAMES = {
@@ -21,9 +26,10 @@ STATUS_NAMES = {
     3: "MACH_ERR_HIP",
     4: "MACH_ERR_NO_DEVICE",
     5: "MACH_ERR_UNSUPPORTED",
+    6: "MACH_ERR_DEBUG_CHECK",
 }
 
-ERR_INVALID_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED = 1, 2, 3, 4, 5
+ERR_INVALID_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_DEBUG_CHECK = 1, 2, 3, 4, 5, 6
 
 # Data-type codes (mach_data_type)
 DATA_F32, DATA_C64, DATA_C32H = 0, 1, 2
@@ -104,7 +110,8 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise NativeLibraryMissing(
             f"mach_amd native library not found at {LIB_PATH}; build it with "
-            "`python mach_amd/build.py` (hipcc, gfx950). There is no CPU fallback."
+            f"`python mach_amd/build.py{' --debug' if DEBUG else ''}` (CMake + ROCm clang, gfx950). "
+            "There is no CPU fallback."
         )
     lib = ctypes.CDLL(LIB_PATH)
     for name, (restype, argtypes) in SIGNATURES.items():

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void das_band_kernel(const DasParams p) {
     for (int j = 0; j < kVoxelsPerWave; ++j) {
         const int64_t slot = (int64_t)tile * kTileVoxels + wave * kVoxelsPerWave + j;
         vok[j] = slot < p.n_vox;
-        vox[j] = (vok[j] && p.order != nullptr) ? (int64_t)p.order[slot] : slot;
+        vox[j] = vok[j] ? voxel_at(p, slot) : slot;
         xv[j] = vok[j] ? (double)p.vox[3 * vox[j] + 0] : 0.0;
         yv[j] = vok[j] ? (double)p.vox[3 * vox[j] + 1] : 0.0;
         zv[j] = vok[j] ? (double)p.vox[3 * vox[j] + 2] : 1.0;
@@ -260,5 +260,9 @@ hipError_t launch_das_band(const DasParams &p, int store, int interp, int fpl, h
         default: return band_interp<kStoreC64>(p, interp, fpl, stream);
     }
 }
+
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_band)
+#endif
 
 }  // namespace mach

@@ -28,4 +28,8 @@ hipError_t section_occupancy0(int *blocks_per_cu) {
 }
 
 }  // namespace batch
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_batch0)
+#endif
+
 }  // namespace mach

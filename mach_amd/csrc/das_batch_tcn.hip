@@ -19,4 +19,8 @@ hipError_t section_occupancy(int section, int *blocks_per_cu) {
 }
 
 }  // namespace batch
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_batch1)
+#endif
+
 }  // namespace mach

@@ -37,6 +37,27 @@ using hvec = _Float16 __attribute__((ext_vector_type(N)));
 
 constexpr int kNearest = 0, kLinear = 1, kQuadratic = 2;
 
+#ifdef MACH_DEBUG
+// Debug builds (MACH_DEBUG): failed bounds checks of this translation unit's kernels, counted instead
+// of trapping (each translation unit holds its own counter; the host reads and clears it after every
+// call through a reader defined with MACH_DEBUG_READER). A failed check also makes the access safe:
+// the pair is skipped, or voxel 0 is used.
+static __device__ unsigned int g_debug_violations;
+__device__ __forceinline__ bool debug_ok(bool cond) {
+    if (!cond) atomicAdd(&g_debug_violations, 1u);
+    return cond;
+}
+#define MACH_DEBUG_READER(name)                                                                          \
+    hipError_t name(uint32_t *count) {                                                                   \
+        const uint32_t zero = 0;                                                                         \
+        hipError_t e = hipMemcpyFromSymbol(count, HIP_SYMBOL(dev::g_debug_violations), sizeof(uint32_t)); \
+        if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(dev::g_debug_violations), &zero, sizeof(uint32_t)); \
+        return e;                                                                                        \
+    }
+#else
+__device__ __forceinline__ constexpr bool debug_ok(bool) { return true; }
+#endif
+
 struct PairSetup {
     int i0;      // first tap sample index, -1 = no contribution
     float frac;  // linear: lerp fraction in [0, 1]; quadratic: x = s - round(s) in [-0.5, 0.5]
@@ -145,11 +166,23 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
         wc = g.w * cs;
         ws = g.w * sn;
     }
+    // Debug builds: every tap of a contributing pair lies in the record (taps tap_step apart; one row
+    // when n_samples == 1).
+    constexpr int kTaps = INTERP == kNearest ? 1 : (INTERP == kLinear ? 2 : 3);
+    if (ok && !debug_ok(i0 >= 0 && (int64_t)i0 + (p.tap_step != 0 ? kTaps : 1) <= p.n_samples)) ok = false;
     r.i0 = ok ? i0 : -1;
     r.frac = ok ? frac : 0.f;
     r.wc = ok ? wc : 0.f;
     r.ws = ok ? ws : 0.f;
     return r;
+}
+
+// The voxel of processing slot `slot` (< n_vox): order[slot], or the slot itself without an order.
+// Debug builds check that the order's entry is a voxel index.
+__device__ __forceinline__ int64_t voxel_at(const DasParams &p, int64_t slot) {
+    int64_t v = p.order != nullptr ? (int64_t)p.order[slot] : slot;
+    if (!debug_ok(v >= 0 && v < p.n_vox)) v = 0;
+    return v;
 }
 
 // Per (voxel, element, transmit) geometry in float64.
@@ -516,7 +549,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
     const int64_t vgroup = (int64_t)bx * 4 + wave;
     const int64_t slot = vgroup * LV + lv;
     const bool vok = slot < p.n_vox;
-    const int64_t v = (vok && p.order != nullptr) ? (int64_t)p.order[slot] : slot;
+    const int64_t v = vok ? voxel_at(p, slot) : slot;
 
     double xv = 0.0, yv = 0.0, zv = 1.0;
     if (vok) {

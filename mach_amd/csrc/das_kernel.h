@@ -118,4 +118,14 @@ bool layout_valid(const DasParams &p, int store, DasLayout l);
 // Enqueues the delay-and-sum kernel. Returns hipSuccess or the launch error.
 hipError_t launch_das(const DasParams &p, int store, int interp, DasLayout layout, hipStream_t stream);
 
+#ifdef MACH_DEBUG
+// Debug builds: failed kernel bounds checks since the last read, per translation unit (read and clear).
+hipError_t debug_violations_f32(uint32_t *count);
+hipError_t debug_violations_c64(uint32_t *count);
+hipError_t debug_violations_c32h(uint32_t *count);
+hipError_t debug_violations_batch0(uint32_t *count);
+hipError_t debug_violations_batch1(uint32_t *count);
+hipError_t debug_violations_band(uint32_t *count);
+#endif
+
 }  // namespace mach

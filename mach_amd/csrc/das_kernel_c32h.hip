@@ -7,4 +7,8 @@ hipError_t launch_das_c32h(const DasParams &p, int interp, DasLayout l, hipStrea
     return launch_das_store<kStoreC32H>(p, interp, l, stream);
 }
 
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_c32h)
+#endif
+
 }  // namespace mach

@@ -7,4 +7,8 @@ hipError_t launch_das_c64(const DasParams &p, int interp, DasLayout l, hipStream
     return launch_das_store<kStoreC64>(p, interp, l, stream);
 }
 
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_c64)
+#endif
+
 }  // namespace mach

@@ -7,4 +7,8 @@ hipError_t launch_das_f32(const DasParams &p, int interp, DasLayout l, hipStream
     return launch_das_store<kStoreF32>(p, interp, l, stream);
 }
 
+#ifdef MACH_DEBUG
+MACH_DEBUG_READER(debug_violations_f32)
+#endif
+
 }  // namespace mach

@@ -87,14 +87,14 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
     for (int q = 0; q < VPL; ++q) {
         const int64_t slot = first + q;
         okq[q] = slot < p.n_vox;
-        vq[q] = okq[q] ? (p.order != nullptr ? (int64_t)p.order[slot] : slot) : 0;
+        vq[q] = okq[q] ? voxel_at(p, slot) : 0;
     }
 
     // Setup role of this lane: voxel q_s of the group, element slot j_s of each round of kEC elements.
     const int q_s = lane / kEC, j_s = lane % kEC;
     const int64_t slot_s = first + q_s;
     const bool vok = slot_s < p.n_vox;
-    const int64_t v = (vok && p.order != nullptr) ? (int64_t)p.order[slot_s] : slot_s;
+    const int64_t v = vok ? voxel_at(p, slot_s) : slot_s;
     double xv = 0.0, yv = 0.0, zv = 1.0, Tv = 0.0;
     if (vok) {
         xv = (double)p.vox[3 * v + 0];

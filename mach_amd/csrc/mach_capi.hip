@@ -177,6 +177,34 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
     return MACH_OK;
 }
 
+// Debug builds (MACH_DEBUG): after each call's launches, wait for them and collect the kernels' failed
+// bounds checks (counted on the device instead of trapping). The counterpart of the reference's
+// CUDA_DEBUG device asserts (kernel.cu:85-92, used at 123, 127, 165-176, 218-230). Release builds
+// return `st` unchanged and add no synchronisation.
+static int debug_check(int st, hipStream_t stream, const char *what) {
+#ifdef MACH_DEBUG
+    if (st != MACH_OK) return st;
+    hipError_t e = hipStreamSynchronize(stream);
+    uint32_t total = 0;
+    for (hipError_t (*read)(uint32_t *) : {mach::debug_violations_f32, mach::debug_violations_c64,
+                                           mach::debug_violations_c32h, mach::debug_violations_batch0,
+                                           mach::debug_violations_batch1, mach::debug_violations_band}) {
+        uint32_t c = 0;
+        if (e == hipSuccess) e = read(&c);
+        total += c;
+    }
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (total != 0)
+        return fail(MACH_ERR_DEBUG_CHECK, std::string("Error (debug build): ") + std::to_string(total) +
+                                              " failed bounds checks in the kernels of " + what);
+    return MACH_OK;
+#else
+    (void)stream;
+    (void)what;
+    return st;
+#endif
+}
+
 // True when a validated request has no term to sum: no voxels, no frames, no transmits or no receive
 // elements. Such a request launches no delay-and-sum kernel and reads none of its inputs (validation
 // lets them be NULL then); the power image still has one (zero) value per voxel.
@@ -250,7 +278,7 @@ int mach_beamform(const mach_beamform_args *args, void *stream) {
         plan.band ? mach::launch_das_band(p, args->data_type, args->interp_type, plan.band_fpl, (hipStream_t)stream)
                   : mach::launch_das(p, args->data_type, args->interp_type, plan.layout, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "beamform kernel launch");
-    return MACH_OK;
+    return debug_check(MACH_OK, (hipStream_t)stream, "mach_beamform");
 }
 
 int mach_batch_create(uint32_t max_requests, int32_t use_graph, mach_batch **batch) {
@@ -284,7 +312,7 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
         e = mach::batch_submit(batch->state, params.data(), stores.data(), interps.data(), layouts.data(), n_requests,
                                (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_batch_submit");
-    return MACH_OK;
+    return debug_check(MACH_OK, (hipStream_t)stream, "mach_batch_submit");
 }
 
 int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_requests, void *stream) {
@@ -306,7 +334,7 @@ int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_req
     if (e == hipSuccess)
         e = mach::batch_submit_noalloc(params, stores, interps, layouts, n_requests, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_batch_submit_noalloc");
-    return MACH_OK;
+    return debug_check(MACH_OK, (hipStream_t)stream, "mach_batch_submit_noalloc");
 }
 
 int mach_batch_destroy(mach_batch *batch) {

@@ -86,6 +86,7 @@ class ServeRequest:
     data_slot: tuple | None = None
     out_slot: tuple | None = None
     error: str | None = None  # why the request failed (it then has no output)
+    service_s: float = float("nan")  # plain path: copy + launch + kernel + synchronisation of this request
 
     @property
     def data_bytes(self) -> int:
@@ -292,6 +293,7 @@ class ServeReport:
     n_fused: int = 0  # requests that ran fused with others of the same geometry
     n_direct: int = 0  # requests (or fused groups' members) launched as their own plain kernel in a batch
     n_failed: int = 0  # requests that failed (validation or launch); not counted in points or latency
+    by_kind: dict = dataclasses.field(default_factory=dict)  # workload -> requests, points, service time
 
     @property
     def gpts_per_s(self) -> float:
@@ -312,6 +314,12 @@ def _report(mode, reqs, t_first, max_batch=1, n_batches=0) -> ServeReport:
                       float(np.percentile(lat, 50)), float(np.percentile(lat, 99)), float(lat.mean()),
                       max_batch, n_batches)
     rep.n_failed = len(reqs) - len(ok)
+    for r in ok:
+        k = rep.by_kind.setdefault(r.kind, {"requests": 0, "points": 0, "service_s": 0.0})
+        k["requests"] += 1
+        k["points"] += r.points
+        if r.service_s == r.service_s:  # set by the plain path only
+            k["service_s"] = round(k["service_s"] + r.service_s, 6)
     return rep
 
 
@@ -604,7 +612,7 @@ def serve_plain(reqs, device, keep_outputs=False):
             time.sleep(dt - 1e-4)
         while time.perf_counter() - t0 < r.arrival_s:
             pass
-        r.admit_s = time.perf_counter() - t0
+        r.admit_s = t_start = time.perf_counter() - t0
         data = torch.empty_like(r.source)
         data.copy_(r.source, non_blocking=True)
         out = torch.empty((r.geometry.scan_coords_m.shape[0], r.source.shape[3]), dtype=torch.complex64,
@@ -615,5 +623,6 @@ def serve_plain(reqs, device, keep_outputs=False):
             r.error, out = str(e), None
         stream.synchronize()
         r.done_s = time.perf_counter() - t0
+        r.service_s = r.done_s - t_start
         r.out = out if keep_outputs else None
     return _report("plain", ordered, ordered[0].arrival_s if ordered else 0.0)

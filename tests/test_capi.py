@@ -198,3 +198,17 @@ def test_noalloc_batch_rejects_before_device_work():
     assert _lib.last_error().startswith("request 1: Error: tukey_alpha")
     empty = (_lib.BeamformArgs * 2)(_args(n_vox=0), _args(n_rx=0, accumulate=1, out=256, scan_coords_m=256))
     assert lib.mach_batch_submit_noalloc(empty, 2, None) == 0, _lib.last_error()
+
+
+def test_debug_library_exports_the_same_entry_points():
+    """libmach_hip_debug.so (the bounds-checked build, MACH_AMD_DEBUG=1) is a drop-in for the release
+    library: same exported functions."""
+    if not os.path.exists(_lib.DEBUG_PATH):
+        pytest.skip("debug build not present (python mach_amd/build.py --debug)")
+
+    def exported(path):
+        out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True)
+        return {line.split()[-1] for line in out.stdout.splitlines() if line.split()[-1].startswith("mach_")}
+
+    assert exported(_lib.DEBUG_PATH) == exported(_lib.RELEASE_PATH)
+    assert set(declared_functions()) <= exported(_lib.DEBUG_PATH)
