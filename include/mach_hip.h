@@ -237,6 +237,14 @@ int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_req
  * std::runtime_error messages (src/mach/kernel.cu:740-769, 926-950). */
 const char *mach_last_error(void);
 
+/* Opt-in timing: with MACH_AMD_PROFILE=1 in the environment (read once per process), every
+ * mach_beamform / mach_batch_submit / mach_batch_submit_noalloc call records HIP events around its
+ * launches on its stream, waits for them, and prints the kernel time and the call's wall time to
+ * stderr. This reads the last timed call's figures on this thread (-1 before any) and returns MACH_OK,
+ * or MACH_ERR_UNSUPPORTED when timing is off. Replaces the reference's compile-time CUDA_PROFILE
+ * section timers and kernel events (src/mach/kernel.cu:31-65, 805-890). */
+int mach_last_timing(float *kernel_ms, float *call_ms);
+
 /* ABI version (MACH_ABI_VERSION). */
 int32_t mach_abi_version(void);
 

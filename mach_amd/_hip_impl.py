@@ -19,6 +19,8 @@ import collections
 import ctypes
 import enum
 import os
+import sys
+import time
 import warnings
 import weakref
 
@@ -274,6 +276,35 @@ def make_args(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, 
     )
 
 
+# MACH_AMD_PROFILE=1: the native library times each call's kernels (mach_last_timing) and this layer
+# times the host<->device staging of CPU arrays, the reference's CUDA_PROFILE sections
+# (kernel.cu:1112, 1146, 1201), printing to stderr.
+_PROFILE = os.environ.get("MACH_AMD_PROFILE", "0") not in ("", "0")
+
+
+def _profile_start(device):
+    if not _PROFILE:
+        return None
+    torch.cuda.synchronize(device)
+    return time.perf_counter()
+
+
+def _profile_end(t0, device, what):
+    if t0 is None:
+        return
+    torch.cuda.synchronize(device)
+    print(f"mach_amd profile: {what} {(time.perf_counter() - t0) * 1e3:.4f} ms", file=sys.stderr, flush=True)
+
+
+def last_timing():
+    """(kernel ms, call ms) of the last timed native call on this thread, or None when timing is off
+    (set MACH_AMD_PROFILE=1 before the library loads)."""
+    k, c = ctypes.c_float(), ctypes.c_float()
+    if _LIB.mach_last_timing(ctypes.byref(k), ctypes.byref(c)) != _lib.MACH_OK:
+        return None
+    return float(k.value), float(c.value)
+
+
 def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampling_freq_hz,
          sound_speed_m_s, modulation_freq_hz, tukey_alpha, interp_type, accumulate_device_out=True, layout=0):
     """Stage CPU arrays if needed, call ``mach_beamform`` and write ``out`` back."""
@@ -293,11 +324,13 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
             UserWarning,
             stacklevel=3,
         )
+    t_stage = _profile_start(device) if n_cpu else None
     on_dev = [t if t.is_cuda else t.to(device) for t in arrays[:4]]
     if any(t.is_cuda and t.device != device for t in arrays):
         raise RuntimeError("All GPU arrays must be on the same device")
     host_out = not out.is_cuda
     dev_out = torch.zeros(out.shape, dtype=out.dtype, device=device) if host_out else out
+    _profile_end(t_stage, device, "stage CPU inputs to the device")
     if layout == 0:
         from . import autotune
 
@@ -316,7 +349,9 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
         status = _LIB.mach_beamform(ctypes.byref(args), ctypes.c_void_p(stream))
         _lib.check(status)
         if host_out:
+            t_copy = _profile_start(device)
             out.copy_(dev_out.cpu())
+            _profile_end(t_copy, device, "copy the result to the host")
         elif _SYNC_EACH_CALL:  # the reference synchronises the device after every call (kernel.cu:874)
             torch.cuda.current_stream(device).synchronize()
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "mach_tx_arrivals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                                         ctypes.c_void_p, ctypes.c_void_p]),
     "mach_last_error": (ctypes.c_char_p, []),
+    "mach_last_timing": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "mach_abi_version": (ctypes.c_int32, []),
     "mach_hip_runtime_version": (ctypes.c_char_p, []),
     "mach_device_arch": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),

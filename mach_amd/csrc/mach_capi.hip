@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <chrono>
 #include <climits>
 #include <cstddef>
 #include <cstdlib>
@@ -53,6 +54,68 @@ int32_t xcd_group_override() {
         return (int32_t)(v < 0 ? 0 : (v > 4096 ? 4096 : v));
     }();
     return g;
+}
+
+// Opt-in timing (MACH_AMD_PROFILE=1, read once per process), the counterpart of the reference's
+// CUDA_PROFILE build (kernel.cu:31-65: host section timers; 805-890: CUDA events around the kernel).
+// A timed call records HIP events around its launches on its stream, waits for them, prints the kernel
+// time and the call's host wall time (which then includes the wait) to stderr and keeps both for
+// mach_last_timing. Off, it costs one branch per call.
+bool profile_enabled() {
+    static const bool on = [] {
+        const char *env = std::getenv("MACH_AMD_PROFILE");
+        return env != nullptr && *env != '\0' && std::strcmp(env, "0") != 0;
+    }();
+    return on;
+}
+
+thread_local float g_last_kernel_ms = -1.f, g_last_call_ms = -1.f;
+
+class CallTimer {
+  public:
+    CallTimer(const char *what, hipStream_t stream) : what_(what), stream_(stream), on_(profile_enabled()) {
+        if (!on_) return;
+        t0_ = std::chrono::steady_clock::now();
+        ok_ = hipEventCreate(&start_) == hipSuccess && hipEventCreate(&stop_) == hipSuccess;
+    }
+    // Marks the start of the call's device work.
+    void launches_begin() {
+        if (on_ && ok_) ok_ = hipEventRecord(start_, stream_) == hipSuccess;
+    }
+    // Marks the end of the call's device work, waits for it and reports; `detail` describes the launch.
+    void launches_end(const std::string &detail) {
+        if (!on_) return;
+        float ms = -1.f;
+        if (ok_ && hipEventRecord(stop_, stream_) == hipSuccess && hipEventSynchronize(stop_) == hipSuccess)
+            hipEventElapsedTime(&ms, start_, stop_);
+        const double call = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+        g_last_kernel_ms = ms;
+        g_last_call_ms = (float)call;
+        std::fprintf(stderr, "mach_amd profile: %s kernel %.4f ms, call %.4f ms; %s\n", what_, ms, call, detail.c_str());
+    }
+    ~CallTimer() {
+        if (start_ != nullptr) hipEventDestroy(start_);
+        if (stop_ != nullptr) hipEventDestroy(stop_);
+    }
+
+  private:
+    const char *what_;
+    hipStream_t stream_;
+    bool on_;
+    bool ok_ = false;
+    hipEvent_t start_ = nullptr, stop_ = nullptr;
+    std::chrono::steady_clock::time_point t0_;
+};
+
+std::string describe(const mach::DasParams &p, int data_type, int interp, const mach::DasLayout &l, bool band) {
+    static const char *kStore[] = {"f32", "c64", "fp16-iq"};
+    static const char *kInterp[] = {"nearest", "linear", "quadratic"};
+    return std::string(band ? "band kernel" : (l.voxels_per_lane > 1 ? "lateral kernel" : "general kernel")) + " " +
+           kStore[data_type] + " " + kInterp[interp] + " lanes/voxel=" + std::to_string(l.lanes_per_voxel) +
+           " frames/lane=" + std::to_string(l.frames_per_lane) + " groups=" + std::to_string(l.chunks_per_lane) +
+           " voxels/lane=" + std::to_string(l.voxels_per_lane) + " n_tx=" + std::to_string(p.n_tx) +
+           " n_rx=" + std::to_string(p.n_rx) + " n_samples=" + std::to_string(p.n_samples) +
+           " n_frames=" + std::to_string(p.n_frames) + " n_vox=" + std::to_string(p.n_vox);
 }
 
 int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
@@ -274,10 +337,13 @@ int mach_beamform(const mach_beamform_args *args, void *stream) {
     Plan plan;
     st = resolve_plan(args, p, &plan);
     if (st != MACH_OK) return st;
+    CallTimer timer("mach_beamform", (hipStream_t)stream);
+    timer.launches_begin();
     const hipError_t e =
         plan.band ? mach::launch_das_band(p, args->data_type, args->interp_type, plan.band_fpl, (hipStream_t)stream)
                   : mach::launch_das(p, args->data_type, args->interp_type, plan.layout, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "beamform kernel launch");
+    if (profile_enabled()) timer.launches_end(describe(p, args->data_type, args->interp_type, plan.layout, plan.band));
     return debug_check(MACH_OK, (hipStream_t)stream, "mach_beamform");
 }
 
@@ -307,11 +373,14 @@ int mach_batch_submit(mach_batch *batch, const mach_beamform_args *requests, uin
     std::vector<mach::DasLayout> layouts(n_requests);
     const int st = build_batch(requests, n_requests, params.data(), stores.data(), interps.data(), layouts.data());
     if (st != MACH_OK) return st;
+    CallTimer timer("mach_batch_submit", (hipStream_t)stream);
+    timer.launches_begin();
     hipError_t e = enqueue_empty_sums(requests, params.data(), n_requests, (hipStream_t)stream);
     if (e == hipSuccess)
         e = mach::batch_submit(batch->state, params.data(), stores.data(), interps.data(), layouts.data(), n_requests,
                                (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_batch_submit");
+    if (profile_enabled()) timer.launches_end(std::to_string(n_requests) + " requests");
     return debug_check(MACH_OK, (hipStream_t)stream, "mach_batch_submit");
 }
 
@@ -330,10 +399,13 @@ int mach_batch_submit_noalloc(const mach_beamform_args *requests, uint32_t n_req
     mach::DasLayout layouts[MACH_NOALLOC_MAX_REQUESTS];
     const int st = build_batch(requests, n_requests, params, stores, interps, layouts);
     if (st != MACH_OK) return st;
+    CallTimer timer("mach_batch_submit_noalloc", (hipStream_t)stream);
+    timer.launches_begin();
     hipError_t e = enqueue_empty_sums(requests, params, n_requests, (hipStream_t)stream);
     if (e == hipSuccess)
         e = mach::batch_submit_noalloc(params, stores, interps, layouts, n_requests, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "mach_batch_submit_noalloc");
+    if (profile_enabled()) timer.launches_end(std::to_string(n_requests) + " requests");
     return debug_check(MACH_OK, (hipStream_t)stream, "mach_batch_submit_noalloc");
 }
 
@@ -440,6 +512,12 @@ int mach_to_db(float *x, uint64_t n, const float *peak, int32_t power_mode, void
 }
 
 const char *mach_last_error(void) { return g_last_error.c_str(); }
+
+int mach_last_timing(float *kernel_ms, float *call_ms) {
+    if (kernel_ms != nullptr) *kernel_ms = g_last_kernel_ms;
+    if (call_ms != nullptr) *call_ms = g_last_call_ms;
+    return profile_enabled() ? MACH_OK : MACH_ERR_UNSUPPORTED;
+}
 
 int32_t mach_abi_version(void) { return MACH_ABI_VERSION; }
 

@@ -110,3 +110,22 @@ def test_no_gpu_fails_loudly():
         beamform(iq, coords, grid, arr, modulation_freq_hz=5e6, **KW)
     with pytest.raises(RuntimeError, match="No HIP device"):
         experimental.beamform(iq[None], coords, grid, arr[None], modulation_freq_hz=5e6, **KW)
+
+
+def test_reference_import_name():
+    """``import mach`` (the reference's package name) gives this package: the same module objects under
+    the reference's dotted names, so code written for the reference imports unchanged."""
+    import importlib
+
+    import mach
+    import mach_amd
+
+    assert mach.beamform is mach_amd.beamform
+    assert mach.InterpolationType is mach_amd.InterpolationType
+    for name in ("kernel", "experimental", "wavefront", "geometry", "io", "io.must"):
+        mod = importlib.import_module(f"mach.{name}")
+        assert mod is importlib.import_module(f"mach_amd.{name}"), name
+    from mach.kernel import nb_beamform  # the raw entry the reference's benchmarks call
+
+    assert nb_beamform is mach_amd.kernel.nb_beamform
+    assert set(["beamform", "geometry", "kernel", "wavefront", "__version__", "Array"]) <= set(mach.__all__)

@@ -212,3 +212,11 @@ def test_debug_library_exports_the_same_entry_points():
 
     assert exported(_lib.DEBUG_PATH) == exported(_lib.RELEASE_PATH)
     assert set(declared_functions()) <= exported(_lib.DEBUG_PATH)
+
+
+def test_timing_is_off_by_default():
+    lib = _lib.load()
+    k, c = ctypes.c_float(), ctypes.c_float()
+    if os.environ.get("MACH_AMD_PROFILE", "0") in ("", "0"):
+        assert lib.mach_last_timing(ctypes.byref(k), ctypes.byref(c)) == _lib.ERR_UNSUPPORTED
+        assert k.value == -1.0 and c.value == -1.0
