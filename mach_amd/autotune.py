@@ -45,6 +45,19 @@ def encode_lateral(voxels_per_lane: int, frames_per_lane: int) -> int:
     return LATERAL_FLAG | ((voxels_per_lane.bit_length() - 1) << 12) | (64 << 4) | frames_per_lane
 
 
+XCD_CODES = (0, 1, 2, 3, 4, 5, 6)  # bits 16-19 of a layout code: automatic, contiguous, runs of 8 .. 128
+
+
+def with_xcd(layout: int, code: int) -> int:
+    """``layout`` with its block-to-XCD remap fixed by ``code`` (bits 16-19: 0 automatic, 1 contiguous
+    eighths, v >= 2 runs of 2^(v+1) blocks)."""
+    return (layout & 0xFFFF) | (code << 16)
+
+
+def xcd_code(layout: int) -> int:
+    return (layout >> 16) & 0xF
+
+
 def decode(layout: int):
     """(lanes_per_voxel, frames_per_lane) of a layout code."""
     return (layout >> 4) & 0xFF, layout & 0xF
@@ -119,14 +132,15 @@ def time_layout(run, layout: int, reps: int = 20, warmup: int = 3) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(run, n_frames: int, reps: int = 20, band: bool = True, lateral: bool = False, rounds: int = 3):
+def tune(run, n_frames: int, reps: int = 20, band: bool = True, lateral: bool = False, rounds: int = 3,
+         cands=None):
     """Times every candidate layout with ``run(layout)``; returns (best_layout, {layout: ms}).
 
     Each candidate is timed in ``rounds`` passes over the list, alternately forwards and backwards, and
     keeps its fastest pass: one short pass per candidate let clock ramps and noise decide between
     layouts within a few percent of each other."""
     times, rejected = {}, set()
-    cands = candidates(n_frames, band=band, lateral=lateral)
+    cands = candidates(n_frames, band=band, lateral=lateral) if cands is None else list(cands)
     for r in range(rounds):
         for lay in cands if r % 2 == 0 else cands[::-1]:
             if lay in rejected:
@@ -253,6 +267,12 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = T
     # the band kernel sums transmits in a different order and has no epilogues: keep it to
     # single-transmit signal requests, so that a tuned layout never changes results
     best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1 and output == 0, lateral=acq.n_tx == 1)
+    # then the winner's block order over the XCDs (it interacts with the voxel order a layout gets, so it
+    # is chosen per layout: e.g. runs of 8 blocks for the 4x4x4-tiled volume, of 32 for Doppler rows)
+    if times[best] > 0 and (best >> 4) & 0xFF != 0:  # not for the band kernel (its own remap)
+        xbest, xtimes = tune(run, acq.n_frames, reps, cands=[with_xcd(best, c) for c in XCD_CODES])
+        times.update(xtimes)
+        best = xbest
     key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox, output)
     if record:
         table().record(key, best, times, {"workload": acq.name, "storage": storage,
@@ -270,7 +290,7 @@ def _main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--power", default="doppler,pymust_disk", help="workloads also tuned for the power Doppler output")
-    ap.add_argument("--volumetric-frames", default="16,32",
+    ap.add_argument("--volumetric-frames", default="",
                     help="extra frame counts of the FP16 volume: the ensembles a server fuses 8-frame volumes into")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)

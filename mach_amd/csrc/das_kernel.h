@@ -78,6 +78,14 @@ constexpr bool multi_group_layout(int lf, int fpl) {
 
 // Layout code of the C ABI: (log2(CH) << 12) | (LF << 4) | FPL; LF == 0 selects the band kernel.
 // Lateral groups: 0x8000 | (log2(VPL) << 12) | (64 << 4) | FPL.
+// Bits 16-19 optionally fix the block-to-XCD remap of the launch (xcd_remap): 0 = chosen per launch
+// (xcd_group_for), 1 = contiguous eighths, v >= 2 = runs of 2^(v + 1) blocks (8 .. 4096).
+// The XCD remap fixed by a layout code's bits 16-19 (the xcd_group of DasParams), or -1 for none.
+inline int32_t layout_xcd_group(int32_t code) {
+    const int v = (code >> 16) & 0xf;
+    return v == 0 ? -1 : (v == 1 ? 0 : (v > 10 ? -1 : 1 << (v + 1)));
+}
+
 inline DasLayout decode_layout(int32_t code) {
     if (code & 0x8000) {
         const int vpl_log2 = (code >> 12) & 0x7;

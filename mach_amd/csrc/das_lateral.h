@@ -154,8 +154,15 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
                     const int start = lo < (int)p.n_samples - kW ? lo : (int)p.n_samples - kW;
                     const char *rs = r0 + (int64_t)start * step;
                     Rows<ST, kW, FPL> w;
+                    // each row: the window's wave-uniform 64-bit base plus a 32-bit lane offset (frame
+                    // offset + row offset, added in 32 bits, which cannot overflow: a channel's block is
+                    // under 4 GiB here), which the loads take directly (scalar base + vector offset). The
+                    // empty asm keeps the compiler from folding the base into 64-bit per-lane addresses.
+                    uint32_t fo_l = fo;
+                    asm volatile("" : "+v"(fo_l));
+                    const uint32_t step32 = (uint32_t)step;
 #pragma unroll
-                    for (int r = 0; r < kW; ++r) w.load(r, rs + r * step + fo);
+                    for (int r = 0; r < kW; ++r) w.load(r, rs + (uint32_t)(fo_l + (uint32_t)r * step32));
 #pragma unroll
                     for (int q = 0; q < VPL; ++q) {
                         if (i0[q] < 0) continue;

@@ -205,7 +205,7 @@ struct Plan {
 static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, Plan *out) {
     out->band = false;
     out->band_fpl = 0;
-    if (a->layout == 0) {  // the band kernel is chosen only through a (tuned) layout
+    if ((a->layout & 0xffff) == 0) {  // the band kernel is chosen only through a (tuned) layout
         out->layout = mach::choose_layout(p, a->data_type);
         return MACH_OK;
     }
@@ -337,6 +337,8 @@ int mach_beamform(const mach_beamform_args *args, void *stream) {
     Plan plan;
     st = resolve_plan(args, p, &plan);
     if (st != MACH_OK) return st;
+    // a remap fixed by the layout code (the autotuner's choice), unless the environment forces one
+    if (p.xcd_group < 0) p.xcd_group = mach::layout_xcd_group(args->layout);
     CallTimer timer("mach_beamform", (hipStream_t)stream);
     timer.launches_begin();
     const hipError_t e =

@@ -103,7 +103,9 @@ def test_shipped_table_entries_are_candidates_of_their_shape():
         n_tx, n_frames, output = int(m.group(3)), int(m.group(5)), int(m.group(7) or 0)
         single = n_tx == 1
         cands = autotune.candidates(n_frames, band=single and output == 0, lateral=single)
-        assert int(entry["layout"]) in cands, (key, hex(int(entry["layout"])))
+        lay = int(entry["layout"])
+        assert lay & 0xFFFF in cands, (key, hex(lay))
+        assert autotune.xcd_code(lay) in autotune.XCD_CODES  # the block order the tuner fixed, if any
         assert entry["layout_hex"] == hex(int(entry["layout"]))
 
 
@@ -127,3 +129,14 @@ def test_nearest_class_fallback():
     assert t.lookup(key(2, 1, 5, 1024, 8, 2097152, output=1)) == 0x44  # output kinds stay apart
     assert t.lookup(key(2, 1, 5, 1024, 8, 1024)) == 0                  # too many octaves away
     assert t.lookup(key(2, 1, 75, 1024, 8, 2097152)) == 0              # other transmit octave
+
+
+def test_xcd_bits_of_layout_codes():
+    """Bits 16-19 of a layout code fix the launch's block order over the XCDs and change nothing else."""
+    lay = autotune.with_xcd(0x1014, 2)
+    assert lay == 0x21014 and autotune.xcd_code(lay) == 2 and autotune.with_xcd(lay, 0) == 0x1014
+    assert autotune.decode(lay) == autotune.decode(0x1014) and autotune.chunks_per_lane(lay) == 2
+    from mach_amd._hip_impl import wave_tile
+
+    assert wave_tile(lay) == wave_tile(0x1014) == 64
+    assert wave_tile(autotune.with_xcd(0, 3)) is None  # XCD bits alone: still the automatic layout
