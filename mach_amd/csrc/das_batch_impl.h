@@ -13,12 +13,25 @@ namespace {
 
 using namespace dev;
 
+// Single-frame requests with transmit chunks factor the rotation, as the plain kernel does for them
+// (das_block's FACT); only the one-frame-per-lane layouts can hold them.
+template <int ST, int INTERP, int LF, int TC>
+__device__ __forceinline__ void run_fpl1(const BatchItem &it, uint32_t bx, uint32_t by) {
+    if constexpr (TC > 1) {
+        if (it.p.n_frames == 1) {
+            das_block<ST, INTERP, LF, 1, TC, 1, false, true>(it.p, bx, by);
+            return;
+        }
+    }
+    das_block<ST, INTERP, LF, 1, TC>(it.p, bx, by);
+}
+
 template <int ST, int INTERP, int TC>
 __device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by) {
     switch (it.layout) {
         case 0: das_block<ST, INTERP, 16, 4, TC>(it.p, bx, by); break;
         case 1: das_block<ST, INTERP, 32, 2, TC>(it.p, bx, by); break;
-        case 2: das_block<ST, INTERP, 64, 1, TC>(it.p, bx, by); break;
+        case 2: run_fpl1<ST, INTERP, 64, TC>(it, bx, by); break;
         case 3: das_block<ST, INTERP, 8, 4, TC>(it.p, bx, by); break;
         case 4: das_block<ST, INTERP, 4, 4, TC>(it.p, bx, by); break;
         case 5: das_block<ST, INTERP, 2, 4, TC>(it.p, bx, by); break;
@@ -26,9 +39,9 @@ __device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uin
         case 7: das_block<ST, INTERP, 16, 2, TC>(it.p, bx, by); break;
         case 8: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by); break;
         case 9: das_block<ST, INTERP, 1, 2, TC>(it.p, bx, by); break;
-        case 10: das_block<ST, INTERP, 16, 1, TC>(it.p, bx, by); break;
-        case 11: das_block<ST, INTERP, 4, 1, TC>(it.p, bx, by); break;
-        default: das_block<ST, INTERP, 1, 1, TC>(it.p, bx, by); break;
+        case 10: run_fpl1<ST, INTERP, 16, TC>(it, bx, by); break;
+        case 11: run_fpl1<ST, INTERP, 4, TC>(it, bx, by); break;
+        default: run_fpl1<ST, INTERP, 1, TC>(it, bx, by); break;
     }
 }
 

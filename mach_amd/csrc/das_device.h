@@ -122,10 +122,30 @@ __device__ __forceinline__ double wave_arrival(const WaveDesc &w, double x, doub
     return (double)(float)(d / w.c + w.offset);
 }
 
+// Unit phasor exp(j 2 pi f t) of a time t (float64): the phase in cycles, f t, is reduced by its
+// nearest integer in float64 (an exact subtraction), rounded to float32 and passed to v_cos_f32 /
+// v_sin_f32, which take revolutions. Measured on gfx950 over [-0.5, 0.5]: max abs error 1.25e-7
+// (rms 3.5e-8) against float64.
+struct Phasor {
+    float c, s;
+};
+
+__device__ __forceinline__ Phasor phasor(double f, double t) {
+    const double cyc = f * t;
+    const float red = (float)(cyc - rint(cyc));
+    return Phasor{__builtin_amdgcn_cosf(red), __builtin_amdgcn_sinf(red)};
+}
+
 // Transmit-dependent part: tau = T[v] + d / c, skips (kernel.cu:634), sample index, interpolation
 // taps (kernel.cu:106-252) and the complex coefficient w * exp(j 2 pi f_mod tau) (kernel.cu:648-661).
-template <int INTERP>
-__device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeom &g, double Tv) {
+// kFactored (transmit chunks): the rotation comes factored as exp(j 2 pi f_mod T[v]) (`a`, per voxel
+// and transmit) times w * exp(j 2 pi f_mod d / c) (`wb`, per voxel and element), so each (pair,
+// transmit) costs one complex product instead of a phase reduction and two transcendentals. The
+// discrete decisions are unchanged (tau, s and the bounds stay float64); the coefficient differs
+// from the direct form by rounding only (two phasors of <= 1.25e-7 error each).
+template <int INTERP, bool kFactored = false>
+__device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeom &g, double Tv, Phasor a = {1.f, 0.f},
+                                               Phasor wb = {0.f, 0.f}) {
     PairSetup r{-1, 0.f, 0.f, 0.f};
     if (g.w == 0.f) return r;  // outside the aperture: often the whole wave, which then skips the rest
     const double tau = Tv + g.dic;
@@ -157,14 +177,14 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
     }
     float wc = g.w, ws = 0.f;
     if (p.rotate) {
-        const double cyc = p.fmod * tau;            // phase in cycles, float64
-        const float red = (float)(cyc - rint(cyc));  // reduced to [-0.5, 0.5] cycles, exact subtraction
-        // v_sin_f32 / v_cos_f32 take revolutions: sin(2 pi red), cos(2 pi red). Measured on gfx950 over
-        // [-0.5, 0.5]: max abs error 1.25e-7 (rms 3.5e-8) against float64.
-        const float sn = __builtin_amdgcn_sinf(red);
-        const float cs = __builtin_amdgcn_cosf(red);
-        wc = g.w * cs;
-        ws = g.w * sn;
+        if constexpr (kFactored) {
+            wc = fmaf(a.c, wb.c, -(a.s * wb.s));
+            ws = fmaf(a.c, wb.s, a.s * wb.c);
+        } else {
+            const Phasor ph = phasor(p.fmod, tau);
+            wc = g.w * ph.c;
+            ws = g.w * ph.s;
+        }
     }
     // Debug builds: every tap of a contributing pair lies in the record (taps tap_step apart; one row
     // when n_samples == 1).
@@ -532,7 +552,13 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
 // WIDE selects 64-bit element offsets for channel blocks of 4 GiB or more (p.bytes32 == 0); otherwise
 // each tap is a wave-uniform 64-bit row base (scalar registers) plus one 32-bit per-lane byte offset,
 // which the loads take directly (scalar-base + vector-offset form, no 64-bit address arithmetic).
-template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false>
+// FACT (transmit chunks of single-frame requests, i.e. compounding; chosen per request by the launcher,
+// so every layout of a request agrees): the rotation factored into a transmit factor per (voxel,
+// transmit) and a receive factor per (voxel, element) (pair_time, kFactored): one complex product per
+// (pair, transmit) instead of a phase reduction and two transcendentals. Measured on the 75-angle
+// compound: 3.47 -> 3.24 ms (same box). Multi-frame requests keep the direct form: the chunk's factors
+// (16 VGPRs) cost the 8-frame volume a wave per SIMD (FP16 20.4 -> 22.3 ms, complex64 32.9 -> 40.9).
+template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false, bool FACT = false>
 __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
     constexpr int LV = 64 / LF;
     constexpr int kGroup = LF * FPL;  // frames per group across the LF lanes of a voxel
@@ -582,17 +608,29 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             // Transmit arrivals are float32 values (array entries, or wave_arrival's rounding), so they
             // are held as float32 and widened exactly where they are used: half the registers.
             float Tv[TC];
+            // transmit chunks: the transmit factor of each voxel's rotation (pair_time, kFactored)
+            constexpr bool kFactor = FACT && TC > 1;
+            Phasor A[kFactor ? TC : 1];
 #pragma unroll
             for (int tt = 0; tt < TC; ++tt) {
                 Tv[tt] = 0.f;
                 if (vok && tt < nt)
                     Tv[tt] = p.waves != nullptr ? (float)wave_arrival(p.waves[t0 + tt], xv, yv, zv)
                                                 : p.arrivals[(t0 + tt) * p.n_vox + v];
+                if constexpr (kFactor) A[tt] = p.rotate ? phasor(p.fmod, (double)Tv[tt]) : Phasor{1.f, 0.f};
             }
             for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
                 const int64_t e = e0 + lf;
                 PairGeom g{0.0, 0.f};
                 if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+                // the receive factor of the rotation, with the apodisation weight (transmit chunks)
+                Phasor wb{0.f, 0.f};
+                if constexpr (kFactor) {
+                    if (p.rotate && g.w != 0.f) {
+                        const Phasor b = phasor(p.fmod, g.dic);
+                        wb = Phasor{g.w * b.c, g.w * b.s};
+                    }
+                }
                 // Multi-transmit chunks whose taps for the whole chunk fit 32 VGPRs (single frames, e.g.
                 // compounding: 4 x 8 dwords) load them together. Wider taps keep one transmit at a time:
                 // for the 8-frame FP16 volume (8 dwords a transmit) groups of 4 cost a wave per SIMD and
@@ -610,7 +648,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
 #pragma unroll
                     for (int tt = 0; tt < TC; ++tt) {
                         s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
-                        if (tt < nt) s[tt] = pair_time<INTERP>(p, g, (double)Tv[tt]);
+                        if (tt < nt) s[tt] = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[kFactor ? tt : 0], wb);
                     }
                 }
                 // Single-transmit quadratic launches: the setup lane forms the Lagrange weights once per
@@ -754,7 +792,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                             if (tt >= nt) break;
                             ElemCoef cf;
                             if constexpr (kLazy) {
-                                const PairSetup st = pair_time<INTERP>(p, g, (double)Tv[tt]);
+                                const PairSetup st = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[tt], wb);
                                 cf = elem_weights<LF, INTERP, false>(st, pw_own, j, lane, st.i0);
                             } else {
                                 cf = elem_coef<LF, INTERP, kPreWeights>(s[tt], pw_own, j, lane);

@@ -15,10 +15,10 @@ namespace {
 
 using namespace dev;
 
-template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false>
+template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false, bool FACT = false>
 __global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE>(p, lb % gridDim.x, lb / gridDim.x);
+    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, FACT>(p, lb % gridDim.x, lb / gridDim.x);
 }
 
 template <int ST, int INTERP, int VPL, int FPL>
@@ -63,6 +63,14 @@ hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     DasParams q = p;
     q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
     if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
+        // single-frame requests (every layout of theirs has one frame per lane) factor the rotation
+        if constexpr (FPL == 1) {
+            if (p.n_frames == 1) {
+                hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE, true>), grid, dim3(256, 1, 1), 0,
+                                   stream, q);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE>), grid, dim3(256, 1, 1), 0, stream, q);
     } else {
         hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, 1, CH, WIDE>), grid, dim3(256, 1, 1), 0, stream, q);
