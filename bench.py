@@ -180,7 +180,7 @@ def cpu_baseline(acq, storage, n_voxels, min_seconds=10.0, seed=0):
     pts = len(idx) * acq.n_rx * acq.n_frames * acq.n_tx
     _, build = das_cpu32.load(native=True)
     rate, runs, total = _time_rate(
-        lambda: das_cpu32.das_cpu32(cd, rx, scan, arr, **acq.params, n_threads=threads), pts, min_seconds)
+        lambda: das_cpu32.das_cpu32(cd, rx, scan, arr, **acq.params, n_threads=threads), pts, min_seconds, 1000)
     orate, oruns, ototal = _time_rate(
         lambda: das_c.das_c(cd, rx, scan, arr, **acq.params, n_threads=threads), pts, min(min_seconds, 3.0), 5)
     return {

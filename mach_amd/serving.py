@@ -449,11 +449,14 @@ def _prepare(members, slots, data_ring, out_ring, stream):
 
 
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
-                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22, hold_min_points=4e9,
-                  max_hold_s=0.5):
+                  fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22, plain_min_points=5e8,
+                  hold_min_points=4e9, max_hold_s=0.5):
     """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
     fused into one ensemble (``fuse``); a request or fused group of at least ``plain_min_voxel_frames``
-    voxel-frames runs as its own plain launch within the batch (``None``: never). With
+    voxel-frames or ``plain_min_points`` points (about 0.1 ms of GPU work: it fills the GPU alone, and
+    its plain launch gets its tuned layout, voxel tiles and XCD order, which the persistent kernel's
+    shared register budget and compiled layout set cannot give it) runs as its own plain launch within
+    the batch (``plain_min_voxel_frames=None``: never). With
     ``hold_min_points``, large requests wait (at most ``max_hold_s`` while the GPU is busy) for
     same-geometry partners to fuse with. Returns a ServeReport; with ``keep_outputs`` each request's
     ``out`` is a private copy of its result."""
@@ -546,7 +549,9 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
                 keep.append(req)
                 if fused is not None:
                     n_fused += len(members)
-                if plain_min_voxel_frames is not None and req.out.numel() >= plain_min_voxel_frames:
+                points = sum(r.points for r in members)
+                if plain_min_voxel_frames is not None and (req.out.numel() >= plain_min_voxel_frames or
+                                                           (plain_min_points is not None and points >= plain_min_points)):
                     direct.append((req, members))
                     n_direct += len(members)
                 else:
