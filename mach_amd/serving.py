@@ -450,7 +450,7 @@ def _prepare(members, slots, data_ring, out_ring, stream):
 
 def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=True, keep_outputs=False,
                   fuse=True, max_fused_frames=256, plain_min_voxel_frames=1 << 22, plain_min_points=5e8,
-                  hold_min_points=4e9, max_hold_s=0.5):
+                  hold_min_points=4e9, max_hold_s=0.5, n_streams=4):
     """Continuous batching over the persistent kernel. Requests of one batch that share a geometry are
     fused into one ensemble (``fuse``); a request or fused group of at least ``plain_min_voxel_frames``
     voxel-frames or ``plain_min_points`` points (about 0.1 ms of GPU work: it fills the GPU alone, and
@@ -458,14 +458,17 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     shared register budget and compiled layout set cannot give it) runs as its own plain launch within
     the batch (``plain_min_voxel_frames=None``: never). With
     ``hold_min_points``, large requests wait (at most ``max_hold_s`` while the GPU is busy) for
-    same-geometry partners to fuse with. Returns a ServeReport; with ``keep_outputs`` each request's
+    same-geometry partners to fuse with. A batch's persistent-kernel submission runs on one of
+    ``n_streams`` compute streams and its direct launches round-robin over all of them, so large
+    requests overlap one another's tails. Returns a ServeReport; with ``keep_outputs`` each request's
     ``out`` is a private copy of its result."""
     free, _total = torch.cuda.mem_get_info(device)
     budget = min(int(budget_bytes), int(free * 0.8))
     data_ring = DeviceRing(budget * 3 // 4, device)
     out_ring = DeviceRing(budget - budget * 3 // 4, device)
     copy_stream = torch.cuda.Stream(device)
-    compute = [torch.cuda.Stream(device), torch.cuda.Stream(device)]
+    compute = [torch.cuda.Stream(device) for _ in range(max(1, n_streams))]
+    next_direct = 0  # round-robin position of direct launches over the compute streams
     runner = _batch.BatchRunner(max_requests=max_batch, use_graph=use_graph)
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival_s))
     waiting = collections.deque()
@@ -523,13 +526,13 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
         marks = still
         # launch: what is admitted (less the large requests held for fusion partners) forms a batch
         take = []
-        if admitted and len(live_batches) < 2:
+        if admitted and len(live_batches) < max(2, len(compute)):
             go, admitted = _select(admitted, len(live_batches) > 0, time.perf_counter() - t0, hold_min_points,
                                    max_hold_s, max_fused_frames)
             take = go[:max_batch]
             admitted = go[max_batch:] + admitted
         if take:
-            stream = compute[n_batches % 2]
+            stream = compute[n_batches % len(compute)]
             for _r, ev in take:
                 stream.wait_event(ev)
             batched, direct, keep, extra = [], [], [], []
@@ -561,14 +564,26 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             # lets those requests complete without waiting for the rest of the batch.
             new_marks = []
 
-            def mark(members):
+            def mark(members, on=None):
                 ev = torch.cuda.Event()
-                ev.record(stream)
+                ev.record(stream if on is None else on)
                 new_marks.append([n_batches, members, ev, None, []])
+                return ev
+
+            # direct launches go round-robin over the compute streams, after the batch stream's work so
+            # far (the copies that fuse groups) and their members' data copies
+            prepared = torch.cuda.Event()
+            prepared.record(stream)
+            direct_done = []
 
             def run_direct(req):
-                _batch.run_plain(req, stream=stream)
-                mark(direct_members[id(req)])
+                nonlocal next_direct
+                st = compute[next_direct % len(compute)]
+                next_direct += 1
+                if st is not stream:
+                    st.wait_event(prepared)
+                _batch.run_plain(req, stream=st)
+                direct_done.append(mark(direct_members[id(req)], on=st))
 
             direct_members = {id(req): members for req, members in direct}
             fails = _submit_isolated(lambda rs: runner.submit(rs, stream=stream), run_direct, batched, direct,
@@ -583,7 +598,10 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
             for members, msg in failed:
                 for r in members:
                     r.error, r.out = msg, None
-            # the final mark: failed members, fused buffers, and every launched request's tensors
+            # the final mark, after every launch of the batch on every stream: failed members, fused
+            # buffers, and every launched request's tensors
+            for ev_d in direct_done:
+                stream.wait_event(ev_d)
             mark([r for members, _ in failed for r in members])
             new_marks[-1][3], new_marks[-1][4] = keep, extra
             marks += new_marks
