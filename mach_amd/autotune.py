@@ -290,6 +290,9 @@ def _main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--power", default="doppler,pymust_disk", help="workloads also tuned for the power Doppler output")
+    ap.add_argument("--envelope", default="doppler,pymust_disk", help="workloads also tuned for the envelope output")
+    ap.add_argument("--outputs", default="signal,power,envelope",
+                    help="output kinds to tune (power and envelope only for the workloads named above)")
     ap.add_argument("--volumetric-frames", default="",
                     help="extra frame counts of the FP16 volume: the ensembles a server fuses 8-frame volumes into")
     args = ap.parse_args()
@@ -301,7 +304,10 @@ def _main():
         for nf in frame_counts:
             acq = phantoms.make(name, device=dev) if nf is None else phantoms.make(name, device=dev, n_frames=nf)
             storages = ("c64", "fp16") if nf is None else ("fp16",)
-            outputs = (0, 1) if name in args.power.split(",") else (0,)
+            kinds = args.outputs.split(",")
+            outputs = [k for k, (kind, names) in enumerate((("signal", None), ("power", args.power),
+                                                             ("envelope", args.envelope)))
+                       if kind in kinds and (names is None or name in names.split(","))]
             reps = args.reps if nf is None else max(3, args.reps * 8 // nf)  # long launches: fewer repetitions
             for st in storages:
                 for output in outputs:
