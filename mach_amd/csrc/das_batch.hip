@@ -50,6 +50,9 @@ int ilog2(int v) {
 // to more frames per lane. Band layouts run as the general kernel's heuristic choice, lateral layouts
 // as the 64-lane layout with their frames per lane.
 DasLayout batch_layout(const DasParams &p, int store, DasLayout want) {
+    // requests whose transmits split the block's waves run as 1 x 1 (the only batched layout compiled
+    // with the split; every layout of theirs sums in the same order)
+    if (tx_split_waves(p.n_tx, p.n_frames) != 1) return DasLayout{1, 1};
     if (want.lanes_per_voxel == 0) want = choose_layout(p, store);
     want.chunks_per_lane = 1;  // the batched kernels walk one frame group per lane
     want.voxels_per_lane = 1;  // and have no lateral groups: a lateral layout runs as its 64-lane layout
@@ -186,7 +189,9 @@ hipError_t pack_table(const DasParams *params, const int *stores, const int *int
             const int idx = layout_index(l);
             if (idx < 0) return hipErrorInvalidValue;
             const int lv = 64 / l.lanes_per_voxel;
-            const int64_t bx = ((p.n_vox + lv - 1) / lv + 3) / 4;
+            // one voxel group per block where the block's waves split the transmits (run_fpl1)
+            const int64_t per_block = 4 / tx_split_waves(p.n_tx, p.n_frames);
+            const int64_t bx = ((p.n_vox + lv - 1) / lv + per_block - 1) / per_block;
             const int64_t by = (p.n_frames + (int64_t)l.lanes_per_voxel * l.frames_per_lane - 1) /
                                ((int64_t)l.lanes_per_voxel * l.frames_per_lane);
             BatchItem &it = items[kept++];

@@ -531,6 +531,12 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
     return r;
 }
 
+// LDS floats das_block needs for the partial sums of a transmit split (4 waves x CH x FPL complex rows).
+template <int CH, int FPL>
+constexpr int tx_part_floats() {
+    return 4 * CH * FPL * 2 * 64;
+}
+
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
 // order) and frame block `by`. Used by the plain kernel and the batched kernel.
 //
@@ -558,8 +564,14 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
 // (pair, transmit) instead of a phase reduction and two transcendentals. Measured on the 75-angle
 // compound: 3.47 -> 3.24 ms (same box). Multi-frame requests keep the direct form: the chunk's factors
 // (16 VGPRs) cost the 8-frame volume a wave per SIMD (FP16 20.4 -> 22.3 ms, complex64 32.9 -> 40.9).
-template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false, bool FACT = false>
-__device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by) {
+// TSX (tx_split_waves of the request: kTxSplitWaves or 1): the workgroup's waves all hold one voxel group, wave w
+// summing transmits [n_tx w / TSX, n_tx (w + 1) / TSX) in chunks of TC; the first wave adds the others'
+// partial sums in wave order through `part` (the caller's LDS, tx_part_floats<CH, FPL>() floats) and
+// writes the result. Summation order per voxel: (transmit range, chunk, element, transmit).
+template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false, bool FACT = false,
+          int TSX = 1>
+__device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by, float *part = nullptr) {
+    static_assert(TSX == 1 || (TSX == 4 && TC > 1), "transmit ranges split 4 waves of a multi-transmit launch");
     constexpr int LV = 64 / LF;
     constexpr int kGroup = LF * FPL;  // frames per group across the LF lanes of a voxel
     constexpr bool kComplex = ST != kStoreF32;
@@ -572,7 +584,9 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lv = lane / LF;
     const int lf = lane % LF;
-    const int64_t vgroup = (int64_t)bx * 4 + wave;
+    const int tw = wave % TSX;  // transmit range of this wave (TSX > 1)
+    const int64_t vgroup = (int64_t)bx * (4 / TSX) + wave / TSX;
+    const int64_t tx_lo = TSX > 1 ? p.n_tx * tw / TSX : 0, tx_hi = TSX > 1 ? p.n_tx * (tw + 1) / TSX : p.n_tx;
     const int64_t slot = vgroup * LV + lv;
     const bool vok = slot < p.n_vox;
     const int64_t v = vok ? voxel_at(p, slot) : slot;
@@ -603,8 +617,8 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
 #pragma unroll
             for (int k = 0; k < FPL; ++k) acc_re[c][k] = acc_im[c][k] = 0.f;
 
-        for (int64_t t0 = 0; t0 < p.n_tx; t0 += TC) {
-            const int nt = (int)((p.n_tx - t0) < TC ? (p.n_tx - t0) : TC);
+        for (int64_t t0 = tx_lo; t0 < tx_hi; t0 += TC) {
+            const int nt = (int)((tx_hi - t0) < TC ? (tx_hi - t0) : TC);
             // Transmit arrivals are float32 values (array entries, or wave_arrival's rounding), so they
             // are held as float32 and widened exactly where they are used: half the registers.
             float Tv[TC];
@@ -833,14 +847,39 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             }
         }
 
+        if constexpr (TSX > 1) {
+            // the other waves' partial sums, added in wave (transmit) order by the group's first wave;
+            // every wave passes both barriers, also where it holds no voxel
+            constexpr int kRows = CH * FPL * 2;  // rows of 64 floats per wave
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+#pragma unroll
+                for (int k = 0; k < FPL; ++k) {
+                    part[(wave * kRows + (c * FPL + k) * 2) * 64 + lane] = acc_re[c][k];
+                    part[(wave * kRows + (c * FPL + k) * 2 + 1) * 64 + lane] = acc_im[c][k];
+                }
+            __syncthreads();
+            if (tw == 0) {
+                for (int w = 1; w < TSX; ++w)
+#pragma unroll
+                    for (int c = 0; c < CH; ++c)
+#pragma unroll
+                        for (int k = 0; k < FPL; ++k) {
+                            acc_re[c][k] += part[((wave + w) * kRows + (c * FPL + k) * 2) * 64 + lane];
+                            acc_im[c][k] += part[((wave + w) * kRows + (c * FPL + k) * 2 + 1) * 64 + lane];
+                        }
+            }
+            __syncthreads();  // the table is rewritten for the next frame block
+        }
 #pragma unroll
         for (int c = 0; c < CH; ++c)
-            if (vok && fok[c]) emit_frames<ST, FPL>(p, v, f0 + (int64_t)c * kGroup, acc_re[c], acc_im[c], pw, vmax);
+            if (vok && fok[c] && tw == 0)
+                emit_frames<ST, FPL>(p, v, f0 + (int64_t)c * kGroup, acc_re[c], acc_im[c], pw, vmax);
     }
 
     if (power) {
         pw = lane_group_sum<LF>(pw);  // lanes lv*LF .. lv*LF+LF-1
-        if (vok && lf == 0) vmax = emit_power(p, v, pw);
+        if (vok && lf == 0 && tw == 0) vmax = emit_power(p, v, pw);
     }
     if (p.output != kOutSignal && p.peak != nullptr) {
         const float m = wave_max(vmax);
