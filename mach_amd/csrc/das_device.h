@@ -146,31 +146,33 @@ __device__ __forceinline__ Phasor phasor(double f, double t) {
 template <int INTERP, bool kFactored = false>
 __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeom &g, double Tv, Phasor a = {1.f, 0.f},
                                                Phasor wb = {0.f, 0.f}) {
-    PairSetup r{-1, 0.f, 0.f, 0.f};
-    if (g.w == 0.f) return r;  // outside the aperture: often the whole wave, which then skips the rest
     const double tau = Tv + g.dic;
     const double s = (tau - p.t0) * p.fs;
     const double ns = (double)p.n_samples;
-    // The other skips (tau < 0, taps outside the record) are one select at the end: as early returns,
-    // each nested exit re-initialised the result under its own exec mask and held that mask in scalar
-    // registers, which spilled in the multi-transmit kernels. Values computed for a skipped pair are
-    // discarded by the select.
-    bool ok = !(tau < 0.0);
+    // Every skip (outside the aperture: w == 0; tau < 0; taps outside the record) is one bit mask at
+    // the end, the tests combined with bitwise ands of 0/1 values and the mask made opaque to the
+    // compiler: as early returns, && chains or selects they became nested exec-mask branches that
+    // re-initialised the result at every level (41 VALU instructions per (pair, transmit) in the
+    // compounding kernel) and held the masks in scalar registers, which spilled. Values computed for
+    // a skipped pair are discarded by the mask. Callers skip a wave none of whose pairs is in the
+    // aperture before calling.
+    unsigned ok = (unsigned)(g.w != 0.f) & (unsigned)!(tau < 0.0);
     int i0;
     float frac = 0.f;
     if constexpr (INTERP == kNearest) {
-        ok = ok && !(s < -0.5 || s > ns - 0.5);
+        ok &= (unsigned)!(s < -0.5) & (unsigned)!(s > ns - 0.5);
         const double k = rint(s);  // round half to even
         i0 = (int)fmin(k, ns - 1.0);  // s in (n-1, n-0.5] would round to n: clamp to the last sample
     } else if constexpr (INTERP == kLinear) {
-        ok = ok && !(s < 0.0 || s > ns - 1.0);
+        ok &= (unsigned)!(s < 0.0) & (unsigned)!(s > ns - 1.0);
         const double fl = floor(s);
         const int i = (int)fl;
-        const bool last = i == p.n_samples - 1 && p.n_samples >= 2;  // s == n-1 exactly: taps (n-2, n-1), t = 1
+        // s == n-1 exactly: taps (n-2, n-1), t = 1 (n_samples < 2^31: tap indices are 32-bit)
+        const bool last = (i == (int)p.n_samples - 1) & (p.n_samples >= 2);
         i0 = last ? i - 1 : i;
         frac = last ? 1.f : (float)(s - fl);
     } else {
-        ok = ok && !(s < 1.0 || s > ns - 2.0);
+        ok &= (unsigned)!(s < 1.0) & (unsigned)!(s > ns - 2.0);
         const double c = rint(s);
         i0 = (int)c - 1;
         frac = (float)(s - c);
@@ -189,11 +191,14 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
     // Debug builds: every tap of a contributing pair lies in the record (taps tap_step apart; one row
     // when n_samples == 1).
     constexpr int kTaps = INTERP == kNearest ? 1 : (INTERP == kLinear ? 2 : 3);
-    if (ok && !debug_ok(i0 >= 0 && (int64_t)i0 + (p.tap_step != 0 ? kTaps : 1) <= p.n_samples)) ok = false;
-    r.i0 = ok ? i0 : -1;
-    r.frac = ok ? frac : 0.f;
-    r.wc = ok ? wc : 0.f;
-    r.ws = ok ? ws : 0.f;
+    if (ok && !debug_ok(i0 >= 0 && (int64_t)i0 + (p.tap_step != 0 ? kTaps : 1) <= p.n_samples)) ok = 0u;
+    int m = -(int)ok;  // all ones for a contributing pair
+    asm volatile("" : "+v"(m));
+    PairSetup r;
+    r.i0 = i0 | ~m;  // -1 when skipped
+    r.frac = __int_as_float(__float_as_int(frac) & m);
+    r.wc = __int_as_float(__float_as_int(wc) & m);
+    r.ws = __int_as_float(__float_as_int(ws) & m);
     return r;
 }
 
@@ -209,7 +214,9 @@ __device__ __forceinline__ int64_t voxel_at(const DasParams &p, int64_t slot) {
 template <int INTERP>
 __device__ __forceinline__ PairSetup pair_setup(const DasParams &p, double xv, double yv, double zv,
                                                 double R2, double inv_R, double Tv, int64_t e) {
-    return pair_time<INTERP>(p, pair_geom(p, xv, yv, zv, R2, inv_R, e), Tv);
+    const PairGeom g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+    if (__ballot(g.w != 0.f) == 0ull) return PairSetup{-1, 0.f, 0.f, 0.f};  // the whole wave outside
+    return pair_time<INTERP>(p, g, Tv);
 }
 
 // Value x of lane j of this lane's aligned group of LF lanes. ds_bpermute runs on the LDS crossbar, off
@@ -659,10 +666,14 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 constexpr bool kLazy = LF == 1 && TC > 1 && kTxGroup == 1;
                 PairSetup s[kLazy ? 1 : TC];
                 if constexpr (!kLazy) {
+                    // a wave none of whose pairs is in the aperture skips the timings (often: elements
+                    // far from the wave's voxels)
+                    const bool any = __ballot(g.w != 0.f) != 0ull;
 #pragma unroll
                     for (int tt = 0; tt < TC; ++tt) {
                         s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
-                        if (tt < nt) s[tt] = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[kFactor ? tt : 0], wb);
+                        if (tt < nt && any)
+                            s[tt] = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[kFactor ? tt : 0], wb);
                     }
                 }
                 // Single-transmit quadratic launches: the setup lane forms the Lagrange weights once per

@@ -127,7 +127,8 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
             const int64_t e = e0 + j_s;
             PairGeom g{0.0, 0.f};
             if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
-            const PairSetup s = pair_time<INTERP>(p, g, Tv);
+            // a round none of whose pairs is in the aperture skips the timing
+            const PairSetup s = __ballot(g.w != 0.f) != 0ull ? pair_time<INTERP>(p, g, Tv) : PairSetup{-1, 0.f, 0.f, 0.f};
             QuadWeights pw_own{0.f, 0.f, 0.f};
             if constexpr (kPre) pw_own = quad_weights(s.frac);
             // element slots some voxel of the group uses (bit q * kEC + j: voxel q, element e0 + j)
