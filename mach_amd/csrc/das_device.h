@@ -93,7 +93,9 @@ __device__ __forceinline__ PairGeom pair_geom(const DasParams &p, double xv, dou
         if (inv_R < 0.0) {
             w = 0.f;
         } else if (!(h2 <= p.flat_r2 * R2)) {
-            const float rr = sqrtf((float)h2) * (float)inv_R;
+            // v_sqrt_f32 directly (1 ulp): the correctly rounded sqrtf adds ~14 instructions of
+            // range scaling and a rounding fix-up, which the continuous taper does not need
+            const float rr = __builtin_amdgcn_sqrtf((float)h2) * (float)inv_R;
             const float sn = __builtin_amdgcn_sinf((1.f - rr) * p.quarter_over_alpha);
             w = sn * sn;
         }
