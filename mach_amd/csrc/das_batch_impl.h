@@ -13,53 +13,45 @@ namespace {
 
 using namespace dev;
 
-// Single-frame requests with transmit chunks factor the rotation, as the plain kernel does for them
-// (das_block's FACT); only the one-frame-per-lane layouts can hold them.
-// With enough transmits (tx_split_waves) the block's 4 waves split one voxel group's transmits, as in the
-// plain kernel; `part` is the batched kernel's LDS for their partial sums. Such requests run as 1 x 1
-// (batch_layout), which sums in the same order as each of their plain layouts.
+// Single-frame requests with enough transmits (tx_split_waves): the block's 4 waves split one voxel
+// group's transmits, as in the plain kernel. Such requests run as 1 x 1 (batch_layout), which sums in
+// the same order as each of their plain layouts.
 template <int ST, int INTERP, int LF, int TC>
-__device__ __forceinline__ void run_fpl1(const BatchItem &it, uint32_t bx, uint32_t by, float *part) {
-    if constexpr (TC > 1) {
-        if (it.p.n_frames == 1) {
-            if constexpr (LF == 1) {
-                if (tx_split_waves(it.p.n_tx, it.p.n_frames) == kTxSplitWaves) {
-                    das_block<ST, INTERP, LF, 1, TC, 1, false, true, kTxSplitWaves>(it.p, bx, by, part);
-                    return;
-                }
-            }
-            das_block<ST, INTERP, LF, 1, TC, 1, false, true>(it.p, bx, by);
+__device__ __forceinline__ void run_fpl1(const BatchItem &it, uint32_t bx, uint32_t by, float *lds) {
+    if constexpr (TC > 1 && LF == 1) {
+        if (tx_split_waves(it.p.n_tx, it.p.n_frames) == kTxSplitWaves) {
+            das_block<ST, INTERP, LF, 1, TC, 1, false, kTxSplitWaves>(it.p, bx, by, lds);
             return;
         }
     }
-    das_block<ST, INTERP, LF, 1, TC>(it.p, bx, by);
+    das_block<ST, INTERP, LF, 1, TC>(it.p, bx, by, lds);
 }
 
 template <int ST, int INTERP, int TC>
-__device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by, float *part) {
+__device__ __forceinline__ void run_layout(const BatchItem &it, uint32_t bx, uint32_t by, float *lds) {
     switch (it.layout) {
-        case 0: das_block<ST, INTERP, 16, 4, TC>(it.p, bx, by); break;
-        case 1: das_block<ST, INTERP, 32, 2, TC>(it.p, bx, by); break;
-        case 2: run_fpl1<ST, INTERP, 64, TC>(it, bx, by, part); break;
-        case 3: das_block<ST, INTERP, 8, 4, TC>(it.p, bx, by); break;
-        case 4: das_block<ST, INTERP, 4, 4, TC>(it.p, bx, by); break;
-        case 5: das_block<ST, INTERP, 2, 4, TC>(it.p, bx, by); break;
-        case 6: das_block<ST, INTERP, 1, 4, TC>(it.p, bx, by); break;
-        case 7: das_block<ST, INTERP, 16, 2, TC>(it.p, bx, by); break;
-        case 8: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by); break;
-        case 9: das_block<ST, INTERP, 1, 2, TC>(it.p, bx, by); break;
-        case 10: run_fpl1<ST, INTERP, 16, TC>(it, bx, by, part); break;
-        case 11: run_fpl1<ST, INTERP, 4, TC>(it, bx, by, part); break;
-        default: run_fpl1<ST, INTERP, 1, TC>(it, bx, by, part); break;
+        case 0: das_block<ST, INTERP, 16, 4, TC>(it.p, bx, by, lds); break;
+        case 1: das_block<ST, INTERP, 32, 2, TC>(it.p, bx, by, lds); break;
+        case 2: run_fpl1<ST, INTERP, 64, TC>(it, bx, by, lds); break;
+        case 3: das_block<ST, INTERP, 8, 4, TC>(it.p, bx, by, lds); break;
+        case 4: das_block<ST, INTERP, 4, 4, TC>(it.p, bx, by, lds); break;
+        case 5: das_block<ST, INTERP, 2, 4, TC>(it.p, bx, by, lds); break;
+        case 6: das_block<ST, INTERP, 1, 4, TC>(it.p, bx, by, lds); break;
+        case 7: das_block<ST, INTERP, 16, 2, TC>(it.p, bx, by, lds); break;
+        case 8: das_block<ST, INTERP, 4, 2, TC>(it.p, bx, by, lds); break;
+        case 9: das_block<ST, INTERP, 1, 2, TC>(it.p, bx, by, lds); break;
+        case 10: run_fpl1<ST, INTERP, 16, TC>(it, bx, by, lds); break;
+        case 11: run_fpl1<ST, INTERP, 4, TC>(it, bx, by, lds); break;
+        default: run_fpl1<ST, INTERP, 1, TC>(it, bx, by, lds); break;
     }
 }
 
 template <int ST, int TC>
-__device__ __forceinline__ void run_interp(const BatchItem &it, uint32_t bx, uint32_t by, float *part) {
+__device__ __forceinline__ void run_interp(const BatchItem &it, uint32_t bx, uint32_t by, float *lds) {
     switch (it.interp) {
-        case 0: run_layout<ST, 0, TC>(it, bx, by, part); break;
-        case 2: run_layout<ST, 2, TC>(it, bx, by, part); break;
-        default: run_layout<ST, 1, TC>(it, bx, by, part); break;
+        case 0: run_layout<ST, 0, TC>(it, bx, by, lds); break;
+        case 2: run_layout<ST, 2, TC>(it, bx, by, lds); break;
+        default: run_layout<ST, 1, TC>(it, bx, by, lds); break;
     }
 }
 
@@ -72,7 +64,9 @@ __global__ __launch_bounds__(256) void das_batch_kernel(const BatchHeader *__res
                                                         uint32_t *counters) {
     constexpr int kSec = TC > 1 ? 1 : 0;
     __shared__ uint32_t s_work;
-    __shared__ float part[TC > 1 ? tx_part_floats<1, 1>() : 1];  // transmit-split partial sums
+    // transmit factors of multi-transmit chunks and transmit-split partial sums (every variant above
+    // needs at most this much: CH = 1, FPL <= 4 with no split, FPL = 1 with it)
+    __shared__ float lds[TC > 1 ? das_lds_floats<TC, 1, 1, kTxSplitWaves>() : 1];
     const uint32_t n_items = header->n_items[kSec];
     const uint32_t total = header->total_blocks[kSec];
     const BatchItem *__restrict__ items = all_items + header->first_item[kSec];
@@ -92,9 +86,9 @@ __global__ __launch_bounds__(256) void das_batch_kernel(const BatchHeader *__res
         const uint32_t local = w - it.first_block;
         const uint32_t bx = local % it.blocks_x, by = local / it.blocks_x;
         switch (it.store) {
-            case kStoreF32: run_interp<kStoreF32, TC>(it, bx, by, part); break;
-            case kStoreC32H: run_interp<kStoreC32H, TC>(it, bx, by, part); break;
-            default: run_interp<kStoreC64, TC>(it, bx, by, part); break;
+            case kStoreF32: run_interp<kStoreF32, TC>(it, bx, by, lds); break;
+            case kStoreC32H: run_interp<kStoreC32H, TC>(it, bx, by, lds); break;
+            default: run_interp<kStoreC64, TC>(it, bx, by, lds); break;
         }
     }
 }

@@ -540,10 +540,11 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
     return r;
 }
 
-// LDS floats das_block needs for the partial sums of a transmit split (4 waves x CH x FPL complex rows).
-template <int CH, int FPL>
-constexpr int tx_part_floats() {
-    return 4 * CH * FPL * 2 * 64;
+// LDS floats das_block needs: the transmit factors of a chunk (4 waves x TC x 64 lanes, complex) and the
+// partial sums of a transmit split (4 waves x CH x FPL complex rows of 64 lanes); at least 1.
+template <int TC, int CH, int FPL, int TSX>
+constexpr int das_lds_floats() {
+    return (TC > 1 ? 4 * TC * 64 * 2 : 0) + (TSX > 1 ? 4 * CH * FPL * 2 * 64 : 0) + (TC > 1 || TSX > 1 ? 0 : 1);
 }
 
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
@@ -567,19 +568,19 @@ constexpr int tx_part_floats() {
 // WIDE selects 64-bit element offsets for channel blocks of 4 GiB or more (p.bytes32 == 0); otherwise
 // each tap is a wave-uniform 64-bit row base (scalar registers) plus one 32-bit per-lane byte offset,
 // which the loads take directly (scalar-base + vector-offset form, no 64-bit address arithmetic).
-// FACT (transmit chunks of single-frame requests, i.e. compounding; chosen per request by the launcher,
-// so every layout of a request agrees): the rotation factored into a transmit factor per (voxel,
-// transmit) and a receive factor per (voxel, element) (pair_time, kFactored): one complex product per
-// (pair, transmit) instead of a phase reduction and two transcendentals. Measured on the 75-angle
-// compound: 3.47 -> 3.24 ms (same box). Multi-frame requests keep the direct form: the chunk's factors
-// (16 VGPRs) cost the 8-frame volume a wave per SIMD (FP16 20.4 -> 22.3 ms, complex64 32.9 -> 40.9).
-// TSX (tx_split_waves of the request: kTxSplitWaves or 1): the workgroup's waves all hold one voxel group, wave w
-// summing transmits [n_tx w / TSX, n_tx (w + 1) / TSX) in chunks of TC; the first wave adds the others'
-// partial sums in wave order through `part` (the caller's LDS, tx_part_floats<CH, FPL>() floats) and
-// writes the result. Summation order per voxel: (transmit range, chunk, element, transmit).
-template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false, bool FACT = false,
-          int TSX = 1>
-__device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by, float *part = nullptr) {
+// Transmit chunks (TC > 1) factor the rotation into a transmit factor per (voxel, transmit) and a
+// receive factor per (voxel, element) (pair_time, kFactored): one complex product per (pair,
+// transmit) instead of a phase reduction and two transcendentals (75-angle compound 3.47 -> 3.24 ms).
+// The chunk's transmit factors live in this lane's own slots of the workgroup's LDS (`lds`, written
+// and read by the same lane, so no barrier): as a register array they cost the 8-frame volume a wave
+// per SIMD (FP16 20.4 -> 22.3 ms).
+// TSX (tx_split_waves of the request: kTxSplitWaves or 1): the workgroup's waves all hold one voxel
+// group, wave w summing transmits [n_tx w / TSX, n_tx (w + 1) / TSX) in chunks of TC; the first wave
+// adds the others' partial sums in wave order through LDS and writes the result. Summation order per
+// voxel: (transmit range, chunk, element, transmit).
+// `lds`: das_lds_floats<TC, CH, FPL, TSX>() floats of the workgroup's LDS.
+template <int ST, int INTERP, int LF, int FPL, int TC = 1, int CH = 1, bool WIDE = false, int TSX = 1>
+__device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint32_t by, float *lds = nullptr) {
     static_assert(TSX == 1 || (TSX == 4 && TC > 1), "transmit ranges split 4 waves of a multi-transmit launch");
     constexpr int LV = 64 / LF;
     constexpr int kGroup = LF * FPL;  // frames per group across the LF lanes of a voxel
@@ -631,17 +632,39 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             // Transmit arrivals are float32 values (array entries, or wave_arrival's rounding), so they
             // are held as float32 and widened exactly where they are used: half the registers.
             float Tv[TC];
-            // transmit chunks: the transmit factor of each voxel's rotation (pair_time, kFactored)
-            constexpr bool kFactor = FACT && TC > 1;
-            Phasor A[kFactor ? TC : 1];
+            // transmit chunks: the transmit factor of each voxel's rotation (pair_time, kFactored). Lanes
+            // with several frames keep them in their LDS slots (slot tt at A_lds[tt * 64]); single-frame
+            // lanes (compounding, whose timings for the chunk are formed together) in registers, which
+            // measured 3 % faster there (2.16 against 2.23 ms). The values are the same either way.
+            constexpr bool kFactor = TC > 1;
+            constexpr bool kALds = kFactor && FPL * CH > 1;
+            float2 *const A_lds = kALds ? (float2 *)lds + wave * TC * 64 + lane : nullptr;
+            Phasor A_reg[kFactor && !kALds ? TC : 1];
 #pragma unroll
             for (int tt = 0; tt < TC; ++tt) {
                 Tv[tt] = 0.f;
                 if (vok && tt < nt)
                     Tv[tt] = p.waves != nullptr ? (float)wave_arrival(p.waves[t0 + tt], xv, yv, zv)
                                                 : p.arrivals[(t0 + tt) * p.n_vox + v];
-                if constexpr (kFactor) A[tt] = p.rotate ? phasor(p.fmod, (double)Tv[tt]) : Phasor{1.f, 0.f};
+                if constexpr (kFactor) {
+                    const Phasor a = p.rotate ? phasor(p.fmod, (double)Tv[tt]) : Phasor{1.f, 0.f};
+                    if constexpr (kALds) {
+                        if (tt < nt) A_lds[tt * 64] = make_float2(a.c, a.s);
+                    } else {
+                        A_reg[tt] = a;
+                    }
+                }
             }
+            auto A_at = [&](int tt) {
+                if constexpr (kALds) {
+                    const float2 a = A_lds[tt * 64];
+                    return Phasor{a.x, a.y};
+                } else if constexpr (kFactor) {
+                    return A_reg[tt];
+                } else {
+                    return Phasor{1.f, 0.f};
+                }
+            };
             for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
                 const int64_t e = e0 + lf;
                 PairGeom g{0.0, 0.f};
@@ -675,7 +698,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                     for (int tt = 0; tt < TC; ++tt) {
                         s[tt] = PairSetup{-1, 0.f, 0.f, 0.f};
                         if (tt < nt && any)
-                            s[tt] = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[kFactor ? tt : 0], wb);
+                            s[tt] = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A_at(tt), wb);
                     }
                 }
                 // Single-transmit quadratic launches: the setup lane forms the Lagrange weights once per
@@ -819,7 +842,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                             if (tt >= nt) break;
                             ElemCoef cf;
                             if constexpr (kLazy) {
-                                const PairSetup st = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A[tt], wb);
+                                const PairSetup st = pair_time<INTERP, kFactor>(p, g, (double)Tv[tt], A_at(tt), wb);
                                 cf = elem_weights<LF, INTERP, false>(st, pw_own, j, lane, st.i0);
                             } else {
                                 cf = elem_coef<LF, INTERP, kPreWeights>(s[tt], pw_own, j, lane);
@@ -864,6 +887,7 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             // the other waves' partial sums, added in wave (transmit) order by the group's first wave;
             // every wave passes both barriers, also where it holds no voxel
             constexpr int kRows = CH * FPL * 2;  // rows of 64 floats per wave
+            float *const part = lds + (TC > 1 ? 4 * TC * 64 * 2 : 0);  // after the transmit factors
 #pragma unroll
             for (int c = 0; c < CH; ++c)
 #pragma unroll

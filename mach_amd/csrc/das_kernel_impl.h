@@ -15,11 +15,11 @@ namespace {
 
 using namespace dev;
 
-template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false, bool FACT = false, int TSX = 1>
+template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false, int TSX = 1>
 __global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
-    __shared__ float part[TSX > 1 ? tx_part_floats<CH, FPL>() : 1];
+    __shared__ float lds[das_lds_floats<TC, CH, FPL, TSX>()];
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, FACT, TSX>(p, lb % gridDim.x, lb / gridDim.x, part);
+    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, TSX>(p, lb % gridDim.x, lb / gridDim.x, lds);
 }
 
 template <int ST, int INTERP, int VPL, int FPL>
@@ -64,20 +64,15 @@ hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     DasParams q = p;
     q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
     if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
-        // single-frame requests (every layout of theirs has one frame per lane) factor the rotation;
-        // with enough transmits the workgroup's 4 waves split one voxel group's transmits
+        // single-frame requests with enough transmits: the workgroup's 4 waves split one voxel group's
+        // transmits (every layout of theirs has one frame per lane)
         if constexpr (FPL == 1) {
-            if (p.n_frames == 1) {
-                if (tx_split_waves(p.n_tx, p.n_frames) == kTxSplitWaves) {
-                    const dim3 grid4((unsigned)n_vgroups, 1, 1);
-                    if (n_vgroups > 0x7fffffff) return hipErrorInvalidConfiguration;
-                    q.xcd_group = (int32_t)xcd_group_for(p, n_vgroups, 1);
-                    hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE, true, kTxSplitWaves>), grid4,
-                                       dim3(256, 1, 1), 0, stream, q);
-                } else {
-                    hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE, true>), grid,
-                                       dim3(256, 1, 1), 0, stream, q);
-                }
+            if (tx_split_waves(p.n_tx, p.n_frames) == kTxSplitWaves) {
+                const dim3 grid4((unsigned)n_vgroups, 1, 1);
+                if (n_vgroups > 0x7fffffff) return hipErrorInvalidConfiguration;
+                q.xcd_group = (int32_t)xcd_group_for(p, n_vgroups, 1);
+                hipLaunchKernelGGL((das_kernel<ST, INTERP, LF, FPL, kTxChunk, CH, WIDE, kTxSplitWaves>), grid4,
+                                   dim3(256, 1, 1), 0, stream, q);
                 return hipGetLastError();
             }
         }
