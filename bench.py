@@ -252,9 +252,13 @@ def _serve_pair(reqs, dev):
 
 def served_stream(n_requests, qps, saturated_qps=None, seed=4):
     """Mixed request stream through the continuous-batching server vs one plain launch per request,
-    at ``qps``; with ``saturated_qps`` the same requests again with their arrival times compressed to
-    that rate, an offered load above what the plain path can serve (the arrivals then no longer bound
-    the makespan, so the ratio measures the server)."""
+    at ``qps`` (latency under Poisson arrivals); with ``saturated_qps`` the same requests again with
+    their arrival times compressed to that rate, an offered load above what the plain path can serve.
+
+    ``throughput_ratio`` at the top is the saturated stream's (batched vs plain points/s over the
+    makespan): at ``qps`` the stream is arrival-bound whenever the plain path keeps up with the
+    arrivals (both paths then finish just after the last request arrives, whatever the server does),
+    which ``arrival_bound`` records for that stream."""
     from mach_amd import serving
 
     dev = torch.device("cuda", 0)
@@ -270,12 +274,20 @@ def served_stream(n_requests, qps, saturated_qps=None, seed=4):
         "last_arrival_s": round(max(r.arrival_s for r in reqs), 4),
     }
     res.update(_serve_pair(reqs, dev))
+    # arrival-bound: the plain path, one request at a time, ends within 10 % of the last arrival
+    res["arrival_bound"] = bool(res["plain"]["makespan_s"] <= 1.1 * res["last_arrival_s"])
+    res["poisson_throughput_ratio"] = res.pop("throughput_ratio")
+    res["throughput_ratio"] = res["poisson_throughput_ratio"]
+    res["throughput_ratio_basis"] = f"Poisson {qps:g} QPS stream"
     if saturated_qps:
         for r in reqs:
             r.arrival_s *= qps / saturated_qps
         sat = {"qps": saturated_qps, "last_arrival_s": round(max(r.arrival_s for r in reqs), 4)}
         sat.update(_serve_pair(reqs, dev))
         res["saturated"] = sat
+        res["throughput_ratio"] = sat["throughput_ratio"]
+        res["throughput_ratio_basis"] = (f"the same requests offered at {saturated_qps:g} QPS, above the plain "
+                                         "path's capacity (points/s over the makespan, batched vs plain)")
     return res
 
 
@@ -390,8 +402,10 @@ def main(argv=None):
     if not args.no_serve:
         log("bench: served mixed stream (batched persistent kernel vs plain per-request)")
         result["served"] = served_stream(args.serve_requests, args.serve_qps, args.serve_saturated_qps or None)
-        log(f"bench: served {result['served']['throughput_ratio']}x plain; "
-            f"p50/p99 {result['served']['batched']['p50_ms']}/{result['served']['batched']['p99_ms']} ms")
+        srv = result["served"]
+        log(f"bench: served at {args.serve_qps:g} QPS {srv['poisson_throughput_ratio']}x plain"
+            f"{' (arrival-bound)' if srv['arrival_bound'] else ''}; "
+            f"p50/p99 {srv['batched']['p50_ms']}/{srv['batched']['p99_ms']} ms")
         if "saturated" in result["served"]:
             sat = result["served"]["saturated"]
             log(f"bench: saturated ({sat['qps']:g} QPS) {sat['throughput_ratio']}x plain; "

@@ -126,9 +126,13 @@ typedef struct mach_beamform_args {
                                          lanes of a voxel are added, so its rounding depends on
                                          lanes per voxel and frames per lane. Bits 16-19 may fix
                                          the launch's block order over the 8 XCDs: 0 = chosen
-                                         per launch, 1 = each XCD a contiguous eighth, v >= 2 =
-                                         runs of 2^(v+1) blocks dealt round-robin (results do
-                                         not depend on it; batches ignore it).                  */
+                                         per launch, 1 = each XCD a contiguous eighth,
+                                         2 <= v <= 10 = runs of 2^(v+1) blocks dealt
+                                         round-robin, 15 = the hardware order (block bx of
+                                         every row on XCD bx % 8; for a voxel_order that gives
+                                         each XCD its own voxels, such as the package's XCD
+                                         wedges of a volume). Results do not depend on it;
+                                         batches ignore it.                                     */
     const int32_t *voxel_order;       /* optional device [n_vox] permutation of 0..n_vox-1: the
                                          order voxels are processed in (NULL = 0, 1, 2, ...).
                                          Results still land at out[v]; a depth-major order

@@ -199,7 +199,85 @@ def _tiled(scan: torch.Tensor, per_wave: int) -> torch.Tensor:
     return torch.argsort(key, stable=True)
 
 
-def voxel_order(scan: torch.Tensor, layout: int = 0):
+# Bits 16-19 of a layout code that select the hardware block order with XCD wedges (kXcdWedgeCode).
+XCD_WEDGE_CODE = 15
+_N_XCD = 8
+
+
+def groups_per_block(n_tx: int, n_frames: int) -> int:
+    """Voxel groups (waves' worth of voxels) per workgroup of a general-kernel launch: 4, or 1 when the
+    workgroup's 4 waves split one group's transmits (one frame and at least 32 transmits;
+    tx_split_waves in das_kernel.h)."""
+    return 1 if (n_frames == 1 and n_tx >= 32) else 4
+
+
+def _xcd_wedges(scan: torch.Tensor, per_wave: int, groups: int):
+    """Wave-sized tiles of a volume grid ordered so that workgroup h (run on XCD h % 8 in the hardware
+    block order) beamforms tiles of that XCD's wedge. The lateral tiles are cut, by angle around the
+    grid's lateral centre, into 8 sectors of equal tile count (on a symmetric grid each sector has the
+    same work at every depth), and XCD x walks sector x of every depth band, band after band (inside a
+    band by distance from the centre, so a workgroup's tiles are neighbours). The interleaved runs of
+    the other remaps balance the depth-dependent work by sending every band through all eight L2s; here
+    each L2 holds only the sample windows of its own sector. Tiles that are not full (grid edges) follow
+    in depth-major order, as in ``_tiled``. Returns None for grids without two lateral axes."""
+    n = scan.shape[0]
+    dev = scan.device
+    ranks = [torch.unique(scan[:, a], return_inverse=True)[1] for a in range(3)]
+    dims = [int(r.max()) + 1 for r in ranks]
+    if dims[0] < 2 or dims[1] < 2:
+        return None
+    sz, sx, sy = _VOLUME_TILES[per_wave]
+    nz, nx, ny = (dims[2] + sz - 1) // sz, (dims[0] + sx - 1) // sx, (dims[1] + sy - 1) // sy
+    lat = (ranks[0] // sx) * ny + ranks[1] // sy
+    tid = (ranks[2] // sz) * (nx * ny) + lat
+    local = ((ranks[2] % sz) * sx + ranks[0] % sx) * sy + ranks[1] % sy
+    count = torch.bincount(tid, minlength=nz * nx * ny).cpu().numpy()
+    # sectors of the lateral tiles: angle around the centre, cut into equal counts, each sector ordered
+    # by distance from the centre. Tiles on a diagonal (equal angles) go to the two sectors it separates
+    # in the pattern A B B A A B B A ... outwards, which gives both the same sum of any work that grows
+    # linearly with the distance.
+    gx = np.repeat(np.arange(nx), ny)
+    gy = np.tile(np.arange(ny), nx)
+    cx, cy = gx - (nx - 1) / 2.0, gy - (ny - 1) / 2.0
+    ang = np.round(np.arctan2(cy, cx), 9)
+    rad = cx * cx + cy * cy
+    ring = np.floor(np.maximum(np.abs(cx), np.abs(cy))).astype(np.int64)
+    sector = np.empty(nx * ny, dtype=np.int64)
+    sector[np.lexsort((rad, (ring + 1) // 2 % 2, ang))] = np.arange(nx * ny) * _N_XCD // (nx * ny)
+    within = np.lexsort((ang, rad, sector))  # by sector, then distance, then angle
+    rank_in = np.empty(nx * ny, dtype=np.int64)
+    rank_in[within] = np.arange(nx * ny) - np.searchsorted(sector[within], sector[within])
+    # each sector's full tiles band after band; workgroup h takes the next `groups` tiles of sector
+    # h % 8 (of the sector with most tiles left once that one is empty, so workgroups stay whole and
+    # sectors of unequal size only move the last workgroups to other XCDs)
+    full = np.nonzero(count == per_wave)[0]
+    f_lat, f_band = full % (nx * ny), full // (nx * ny)
+    f_sec = sector[f_lat]
+    o = np.lexsort((rank_in[f_lat], f_band, f_sec))
+    lists = np.split(full[o], np.searchsorted(f_sec[o], np.arange(1, _N_XCD)))
+    left = np.array([len(t) for t in lists])
+    take = np.zeros(_N_XCD, dtype=np.int64)
+    seq = []
+    h = 0
+    while left.sum():
+        for _ in range(groups):
+            x = h % _N_XCD if left[h % _N_XCD] else int(np.argmax(left))
+            if not left[x]:
+                break
+            seq.append(lists[x][take[x]])
+            take[x] += 1
+            left[x] -= 1
+        h += 1
+    tile_pos = torch.full((nz * nx * ny,), -1, dtype=torch.int64)
+    tile_pos[torch.as_tensor(np.array(seq, dtype=np.int64))] = torch.arange(len(seq))
+    tile_pos = tile_pos.to(dev)[tid]
+    dm_rank = torch.empty(n, dtype=torch.int64, device=dev)
+    dm_rank[_depth_major(scan)] = torch.arange(n, device=dev)
+    key = torch.where(tile_pos >= 0, tile_pos * per_wave + local, len(full) * per_wave + dm_rank)
+    return torch.argsort(key, stable=True)
+
+
+def voxel_order(scan: torch.Tensor, layout: int = 0, groups: int = 4):
     """Processing order of the voxels for a launch with this layout code.
 
     Depth-major (sorted by z, then x, then y; stable) by default: waves that run together beamform
@@ -207,24 +285,30 @@ def voxel_order(scan: torch.Tensor, layout: int = 0):
     so those rows stay in L2. For general-kernel layouts whose waves hold 16 or more voxels
     (``wave_tile``), wave-sized tiles instead (``_tiled``): the elements a wave visits are the union of
     its voxels' apertures, so a compact tile wastes fewer lanes than a row, and its sample windows
-    overlap more. Any permutation gives the same results (each voxel is computed independently, with a
-    fixed summation order); the order only affects speed. Cached per scan-coordinate tensor (identity
-    and in-place version) and tile. ``MACH_AMD_VOXEL_ORDER=0`` disables it (caller order),
-    ``MACH_AMD_VOXEL_ORDER=rows`` keeps depth-major rows for every layout.
+    overlap more. Layouts whose bits 16-19 select XCD wedges (``XCD_WEDGE_CODE``) get those tiles in
+    XCD wedges on volume grids (``_xcd_wedges``, for launches of ``groups`` voxel groups per workgroup:
+    ``groups_per_block``). Any permutation gives the same results (each voxel is computed
+    independently, with a fixed summation order); the order only affects speed. Cached per
+    scan-coordinate tensor (identity and in-place version), tile and wedge mode.
+    ``MACH_AMD_VOXEL_ORDER=0`` disables it (caller order), ``MACH_AMD_VOXEL_ORDER=rows`` keeps
+    depth-major rows for every layout.
     """
     n = scan.shape[0]
     mode = os.environ.get("MACH_AMD_VOXEL_ORDER", "1")
     if n < _ORDER_MIN_VOXELS or not scan.is_cuda or mode == "0":
         return None
     per_wave = None if mode == "rows" else wave_tile(layout)
-    key = (id(scan), scan.data_ptr(), n, per_wave)
+    wedges = groups if per_wave is not None and (layout >> 16) & 0xF == XCD_WEDGE_CODE else 0
+    key = (id(scan), scan.data_ptr(), n, per_wave, wedges)
     hit = _ORDER_CACHE.get(key)
     if hit is not None:
         ref, version, order = hit
         if ref() is scan and version == scan._version:
             _ORDER_CACHE.move_to_end(key)
             return order
-    idx = _depth_major(scan) if per_wave is None else _tiled(scan, per_wave)
+    idx = _xcd_wedges(scan, per_wave, wedges) if wedges else None
+    if idx is None:
+        idx = _depth_major(scan) if per_wave is None else _tiled(scan, per_wave)
     order = idx.to(torch.int32).contiguous()
     _ORDER_CACHE[key] = (weakref.ref(scan), scan._version, order)
     while len(_ORDER_CACHE) > _ORDER_CACHE_SIZE:
@@ -337,7 +421,7 @@ def _run(cd, rx, scan, arrivals, out, *, n_tx, kind, f_number, rx_start_s, sampl
         core = _core_shape(on_dev[0], kind)
         layout = autotune.layout_for(kind, int(interp_type), n_tx, core[-3], dev_out.shape[1], on_dev[2].shape[0],
                                      on_dev[0].data_ptr(), dev_out.data_ptr())
-    order = voxel_order(on_dev[2], layout)
+    order = voxel_order(on_dev[2], layout, groups_per_block(n_tx, dev_out.shape[1]))
     args = make_args(*on_dev, dev_out, n_tx=n_tx, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
                      sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
                      modulation_freq_hz=modulation_freq_hz, tukey_alpha=tukey_alpha, interp_type=interp_type,

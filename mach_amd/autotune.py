@@ -45,12 +45,14 @@ def encode_lateral(voxels_per_lane: int, frames_per_lane: int) -> int:
     return LATERAL_FLAG | ((voxels_per_lane.bit_length() - 1) << 12) | (64 << 4) | frames_per_lane
 
 
-XCD_CODES = (0, 1, 2, 3, 4, 5, 6)  # bits 16-19 of a layout code: automatic, contiguous, runs of 8 .. 128
+# bits 16-19 of a layout code: automatic, contiguous, runs of 8 .. 128, and XCD wedges (the hardware
+# block order over a voxel order that gives each XCD one angular sector of every depth band of a volume)
+XCD_CODES = (0, 1, 2, 3, 4, 5, 6, 15)
 
 
 def with_xcd(layout: int, code: int) -> int:
     """``layout`` with its block-to-XCD remap fixed by ``code`` (bits 16-19: 0 automatic, 1 contiguous
-    eighths, v >= 2 runs of 2^(v+1) blocks)."""
+    eighths, 2 <= v <= 10 runs of 2^(v+1) blocks, 15 XCD wedges)."""
     return (layout & 0xFFFF) | (code << 16)
 
 
@@ -255,7 +257,8 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = T
     lib = _lib.load()
 
     def run(layout):
-        order = _hip_impl.voxel_order(acq.scan_coords_m, layout)  # each candidate with its own order
+        # each candidate with its own order
+        order = _hip_impl.voxel_order(acq.scan_coords_m, layout, _hip_impl.groups_per_block(acq.n_tx, acq.n_frames))
         args = _hip_impl.make_args(cd, acq.rx_coords_m, acq.scan_coords_m, acq.tx_wave_arrivals_s, out,
                                    n_tx=acq.n_tx, kind=kind, f_number=p["f_number"], rx_start_s=p["rx_start_s"],
                                    sampling_freq_hz=p["sampling_freq_hz"], sound_speed_m_s=p["sound_speed_m_s"],
@@ -270,7 +273,9 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = T
     # then the winner's block order over the XCDs (it interacts with the voxel order a layout gets, so it
     # is chosen per layout: e.g. runs of 8 blocks for the 4x4x4-tiled volume, of 32 for Doppler rows)
     if times[best] > 0 and (best >> 4) & 0xFF != 0:  # not for the band kernel (its own remap)
-        xbest, xtimes = tune(run, acq.n_frames, reps, cands=[with_xcd(best, c) for c in XCD_CODES])
+        # XCD wedges only where the layout has wave tiles (the order of the other layouts ignores them)
+        codes = [c for c in XCD_CODES if c != _hip_impl.XCD_WEDGE_CODE or _hip_impl.wave_tile(best) is not None]
+        xbest, xtimes = tune(run, acq.n_frames, reps, cands=[with_xcd(best, c) for c in codes])
         times.update(xtimes)
         best = xbest
     key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox, output)
@@ -294,7 +299,7 @@ def _main():
     ap.add_argument("--outputs", default="signal,power,envelope",
                     help="output kinds to tune (power and envelope only for the workloads named above)")
     ap.add_argument("--volumetric-frames", default="",
-                    help="extra frame counts of the FP16 volume: the ensembles a server fuses 8-frame volumes into")
+                    help="extra frame counts of the FP16 volume to tune (longer volumetric ensembles)")
     args = ap.parse_args()
     dev = torch.device("cuda", 0)
     for name in args.workloads.split(","):

@@ -90,11 +90,23 @@ constexpr bool multi_group_layout(int lf, int fpl) {
 // Layout code of the C ABI: (log2(CH) << 12) | (LF << 4) | FPL; LF == 0 selects the band kernel.
 // Lateral groups: 0x8000 | (log2(VPL) << 12) | (64 << 4) | FPL.
 // Bits 16-19 optionally fix the block-to-XCD remap of the launch (xcd_remap): 0 = chosen per launch
-// (xcd_group_for), 1 = contiguous eighths, v >= 2 = runs of 2^(v + 1) blocks (8 .. 4096).
+// (xcd_group_for), 1 = contiguous eighths, 2 <= v <= 10 = runs of 2^(v + 1) blocks (8 .. 4096),
+// kXcdWedgeCode = the hardware order itself (runs of 1 block: block bx of every block row on XCD
+// bx % 8, the rows padded to a multiple of 8 blocks), for a voxel order that already assigns each XCD
+// its own voxels (the host's XCD wedges of a volume: every XCD one angular sector of every depth band).
+constexpr int kXcdWedgeCode = 15;
 // The XCD remap fixed by a layout code's bits 16-19 (the xcd_group of DasParams), or -1 for none.
 inline int32_t layout_xcd_group(int32_t code) {
     const int v = (code >> 16) & 0xf;
+    if (v == kXcdWedgeCode) return 1;
     return v == 0 ? -1 : (v == 1 ? 0 : (v > 10 ? -1 : 1 << (v + 1)));
+}
+
+// Blocks per row of a launch under remap `group`: with the hardware order (runs of 1 block) and
+// several rows, each row is padded to a multiple of the 8 XCDs, so that block bx of every row runs on
+// XCD bx % 8 (the padding blocks exit at once).
+inline int64_t padded_blocks_x(int64_t blocks_x, int64_t blocks_y, uint32_t group) {
+    return (group == 1u && blocks_y > 1) ? (blocks_x + 7) / 8 * 8 : blocks_x;
 }
 
 inline DasLayout decode_layout(int32_t code) {

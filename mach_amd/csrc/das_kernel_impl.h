@@ -19,25 +19,29 @@ template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false
 __global__ __launch_bounds__(256) void das_kernel(const DasParams p) {
     __shared__ float lds[das_lds_floats<TC, CH, FPL, TSX>()];
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, TSX>(p, lb % gridDim.x, lb / gridDim.x, lds);
+    const uint32_t bx = lb % gridDim.x;
+    if ((int64_t)bx * (4 / TSX) * (64 / LF) >= p.n_vox) return;  // row padding (padded_blocks_x)
+    das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, TSX>(p, bx, lb / gridDim.x, lds);
 }
 
 template <int ST, int INTERP, int VPL, int FPL>
 __global__ __launch_bounds__(256) void das_lateral_kernel(const DasParams p) {
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    das_lateral_block<ST, INTERP, VPL, FPL>(p, lb % gridDim.x, lb / gridDim.x);
+    const uint32_t bx = lb % gridDim.x;
+    if ((int64_t)bx * 4 * VPL >= p.n_vox) return;  // row padding (padded_blocks_x)
+    das_lateral_block<ST, INTERP, VPL, FPL>(p, bx, lb / gridDim.x);
 }
 
 // Lateral groups of VPL voxels per wave, 64 * FPL frames per block (single transmit).
 template <int ST, int INTERP, int VPL, int FPL>
 hipError_t launch_lateral(const DasParams &p, hipStream_t stream) {
     const int64_t n_groups = (p.n_vox + VPL - 1) / VPL;
-    const int64_t blocks_x = (n_groups + 3) / 4;
     const int64_t per = 64 * FPL;
     const int64_t blocks_y = p.output == kOutPower ? 1 : (p.n_frames + per - 1) / per;
-    if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
     DasParams q = p;
-    q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
+    q.xcd_group = (int32_t)xcd_group_for(p, (n_groups + 3) / 4, blocks_y);
+    const int64_t blocks_x = padded_blocks_x((n_groups + 3) / 4, blocks_y, (uint32_t)q.xcd_group);
+    if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((das_lateral_kernel<ST, INTERP, VPL, FPL>), dim3((unsigned)blocks_x, (unsigned)blocks_y, 1),
                        dim3(256, 1, 1), 0, stream, q);
     return hipGetLastError();
@@ -55,14 +59,14 @@ template <int ST, int INTERP, int LF, int FPL, int CH, bool WIDE = false>
 hipError_t launch_one(const DasParams &p, hipStream_t stream) {
     constexpr int LV = 64 / LF;
     const int64_t n_vgroups = (p.n_vox + LV - 1) / LV;
-    const int64_t blocks_x = (n_vgroups + 3) / 4;
     // the power epilogue walks all frame blocks inside each block
     const int64_t per = (int64_t)LF * FPL * CH;
     const int64_t blocks_y = p.output == kOutPower ? 1 : (p.n_frames + per - 1) / per;
+    DasParams q = p;
+    q.xcd_group = (int32_t)xcd_group_for(p, (n_vgroups + 3) / 4, blocks_y);
+    const int64_t blocks_x = padded_blocks_x((n_vgroups + 3) / 4, blocks_y, (uint32_t)q.xcd_group);
     if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks_x, (unsigned)blocks_y, 1);
-    DasParams q = p;
-    q.xcd_group = (int32_t)xcd_group_for(p, blocks_x, blocks_y);
     if (p.n_tx > 1) {  // amortise the pair geometry over chunks of transmits
         // single-frame requests with enough transmits: the workgroup's 4 waves split one voxel group's
         // transmits (every layout of theirs has one frame per lane)

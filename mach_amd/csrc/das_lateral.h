@@ -14,7 +14,9 @@
 // scalar register, so the window position and each voxel's offset in it are scalar branches (its
 // weights arrive by ds_bpermute, off the VALU).
 // Each voxel and frame is summed element by element with the general kernel's float operations
-// (combine_rows), so the result is bit-identical to every other layout. Single-transmit launches only.
+// (combine_rows), so the signal and envelope outputs are bit-identical to every other layout's (the power
+// image sums each lane's frames first, so it matches layouts with the same frames per lane). Single-transmit
+// launches only.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -27,7 +27,8 @@ import torch
 
 from . import _lib
 from ._hip_impl import (OUTPUT_ENVELOPE, OUTPUT_POWER, InterpolationType, _as_tensor, _core_shape, _data_kind,
-                        _device_for, _require, _shape_str, check_dimensions, make_args, order_on, voxel_order)
+                        _device_for, _require, _shape_str, check_dimensions, groups_per_block, make_args, order_on,
+                        voxel_order)
 
 __all__ = ["power_doppler", "envelope", "bmode_db", "power_doppler_db", "to_db"]
 
@@ -87,7 +88,8 @@ def _launch(cd, rx, scan, arr, out, peak, *, kind, n_tx, output, accumulate, f_n
                      sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=sound_speed_m_s,
                      modulation_freq_hz=fmod, tukey_alpha=tukey_alpha, interp_type=interp_type,
                      accumulate=accumulate, layout=layout,
-                     order=order_on(voxel_order(scan, layout), torch.cuda.current_stream(device)), output=output,
+                     order=order_on(voxel_order(scan, layout, groups_per_block(n_tx, core[-1])),
+                                    torch.cuda.current_stream(device)), output=output,
                      peak=peak)
     lib = _lib.load()
     with torch.cuda.device(device):

@@ -61,14 +61,15 @@ class Geometry:
     params: dict
     orders: dict = dataclasses.field(default_factory=dict)  # voxel order per wave tile, held for the geometry
 
-    def order_for(self, layout: int):
-        """The voxel processing order for a launch with this layout code, computed once per wave tile and
-        held by the geometry: batches in flight on other streams read it, so it must outlive them (the
-        library's own order cache is bounded and may drop it)."""
-        per_wave = _hip_impl.wave_tile(layout)
-        if per_wave not in self.orders:
-            self.orders[per_wave] = voxel_order(self.scan_coords_m, layout)
-        return self.orders[per_wave]
+    def order_for(self, layout: int, groups: int = 4):
+        """The voxel processing order for a launch with this layout code (and ``groups`` voxel groups per
+        workgroup), computed once per wave tile and wedge mode and held by the geometry: batches in flight
+        on other streams read it, so it must outlive them (the library's own order cache is bounded and may
+        drop it)."""
+        key = (_hip_impl.wave_tile(layout), (layout >> 16) & 0xF == _hip_impl.XCD_WEDGE_CODE, groups)
+        if key not in self.orders:
+            self.orders[key] = voxel_order(self.scan_coords_m, layout, groups)
+        return self.orders[key]
 
 
 @dataclasses.dataclass
@@ -275,7 +276,7 @@ def _to_request(req: ServeRequest, data: torch.Tensor, out: torch.Tensor) -> _ba
                        voxel_order=g.voxel_order)
     r.layout = autotune.layout_for(r.kind(), r.interp_type, r.n_tx(), g.rx_coords_m.shape[0], out.shape[1],
                                    g.scan_coords_m.shape[0], data.data_ptr(), out.data_ptr())
-    r.voxel_order = g.order_for(r.layout)
+    r.voxel_order = g.order_for(r.layout, _hip_impl.groups_per_block(r.n_tx(), out.shape[1]))
     return r
 
 

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from ._hip_impl import (InterpolationType, _as_tensor, _core_shape, _data_kind, _require, _shape_str,
-                        check_dimensions, make_args, order_on, voxel_order)
+                        check_dimensions, groups_per_block, make_args, order_on, voxel_order)
 
 __all__ = ["PlaneWave", "SphericalWave", "pack", "arrivals", "beamform"]
 
@@ -159,7 +159,8 @@ def beamform(channel_data, rx_coords_m, scan_coords_m, waves, out=None, *, rx_st
     args = make_args(cd, rx, scan, None, o, n_tx=n_tx, kind=kind, f_number=f_number, rx_start_s=rx_start_s,
                      sampling_freq_hz=sampling_freq_hz, sound_speed_m_s=c, modulation_freq_hz=fmod,
                      tukey_alpha=tukey_alpha, interp_type=interp_type, accumulate=accumulate, layout=layout,
-                     order=order_on(voxel_order(scan, layout), torch.cuda.current_stream(cd.device)),
+                     order=order_on(voxel_order(scan, layout, groups_per_block(n_tx, core[-1])),
+                                    torch.cuda.current_stream(cd.device)),
                      tx_waves=packed)
     lib = _lib.load()
     with torch.cuda.device(cd.device):

@@ -6,7 +6,7 @@ import pytest
 import torch
 
 from mach_amd import probes
-from mach_amd._hip_impl import _depth_major, _tiled, wave_tile
+from mach_amd._hip_impl import _depth_major, _tiled, _xcd_wedges, wave_tile
 
 
 def grid(nx, ny, nz):
@@ -64,3 +64,64 @@ def test_scattered_cloud_falls_back_to_depth_major():
     order = _tiled(scan, 32)
     assert is_perm(order, 5000)
     assert torch.equal(order, _depth_major(scan))  # no tile of a random cloud is full
+
+
+def _ranks(scan, order):
+    pts = scan[order].numpy()
+    return np.stack([np.searchsorted(np.unique(pts[:, a]), pts[:, a]) for a in range(3)], -1)
+
+
+@pytest.mark.parametrize("shape,per_wave,groups", [((64, 64, 64), 64, 4), ((80, 80, 80), 64, 4),
+                                                   ((64, 64, 32), 32, 4), ((48, 48, 16), 16, 1),
+                                                   ((30, 26, 22), 64, 4)])
+def test_xcd_wedges_give_each_xcd_one_sector(shape, per_wave, groups):
+    """Workgroup h runs on XCD h % 8 in the hardware block order: over the full tiles, XCD x's voxels lie
+    in one angular sector of the lateral plane, every sector holds the same number of tiles of every
+    depth band, an XCD walks the bands in order, and the grid's partial tiles follow."""
+    scan = grid(*shape)
+    n = scan.shape[0]
+    order = _xcd_wedges(scan, per_wave, groups)
+    assert is_perm(order, n)
+    tz, tx, ty = {64: (4, 4, 4), 32: (2, 4, 4), 16: (2, 2, 4)}[per_wave]
+    nxt, nyt, nzt = shape[0] // tx, shape[1] // ty, shape[2] // tz
+    n_full = nxt * nyt * nzt
+    r = _ranks(scan, order)
+    wg = per_wave * groups
+    cx, cy = (shape[0] - 1) / 2.0, (shape[1] - 1) / 2.0
+    tiles = [[] for _ in range(8)]
+    for k in range(n_full):
+        t = r[k * per_wave:(k + 1) * per_wave]
+        assert tuple(t.max(0) - t.min(0) + 1) == (tx, ty, tz), k  # still one whole wave tile
+        tiles[(k * per_wave // wg) % 8].append((int(t[0, 2]) // tz, float(np.arctan2(t[:, 1].mean() - cy,
+                                                                                    t[:, 0].mean() - cx))))
+    equal = nxt * nyt % 8 == 0 and nxt % 2 == 0 and nyt % 2 == 0
+    for x in range(8):
+        bands = [b for b, _ in tiles[x]]
+        if equal:
+            assert bands == sorted(bands)  # band after band
+            assert np.bincount(bands).tolist() == [nxt * nyt // 8] * nzt  # equal tiles per band
+            a = np.array([ang for _, ang in tiles[x]])
+            lo, hi = -np.pi + x * np.pi / 4, -np.pi + (x + 1) * np.pi / 4
+            assert np.all((a >= lo - 1e-9) & (a <= hi + 1e-9)), x  # one sector of pi/4 (diagonals shared)
+    # partial tiles (grid edges) follow the full ones in depth-major order
+    rest = order[n_full * per_wave:]
+    if len(rest):
+        assert torch.equal(rest, torch.tensor([i for i in _depth_major(scan).tolist() if i in set(rest.tolist())]))
+
+
+def test_xcd_wedges_sectors_are_balanced_on_a_symmetric_grid():
+    """On a square grid centred on the aperture the eight sectors are images of each other under the
+    square's symmetries (diagonal tiles alternate between neighbours), so any work that depends on the
+    distance from the centre is the same per XCD: here the summed distance of each XCD's tiles."""
+    scan = grid(128, 128, 8)
+    order = _xcd_wedges(scan, 64, 4)
+    r = _ranks(scan, order)
+    per_tile = r.reshape(-1, 64, 3).mean(1)
+    xcd = (np.arange(per_tile.shape[0]) // 4) % 8
+    d = np.hypot(per_tile[:, 0] - 63.5, per_tile[:, 1] - 63.5)
+    sums = np.array([d[xcd == x].sum() for x in range(8)])
+    assert np.ptp(sums) / sums.mean() < 2e-3, sums
+
+
+def test_xcd_wedge_order_needs_two_lateral_axes():
+    assert _xcd_wedges(grid(64, 1, 64), 64, 4) is None  # planar grids keep their tiles
