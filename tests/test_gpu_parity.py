@@ -24,6 +24,7 @@ pytestmark = pytest.mark.gpu
 TOL_FP32 = 1e-5
 TOL_FP16_SAME_DATA = 1e-5
 TOL_FP16_END_TO_END = 1e-3
+TOL_BMODE_DB = 0.1  # B-mode (envelope, dB) difference of FP16 storage against FP32 data, over -60 dB
 
 
 def log(msg):
@@ -264,6 +265,7 @@ def test_workloads_fp16_storage(device, name, scale, sample):
     db = bmode_db_diff(np.abs(y16).mean(-1), np.abs(ref32).mean(-1))
     log(f"  {name} fp16: rel_l2 same-data={e_same:.2e} end-to-end={e_e2e:.2e} B-mode dB diff={db:.3f}")
     assert e_same <= TOL_FP16_SAME_DATA and e_e2e <= TOL_FP16_END_TO_END
+    assert db <= TOL_BMODE_DB
 
 
 @pytest.mark.parametrize("frames", [1, 8, 12, 256])
