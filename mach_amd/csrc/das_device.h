@@ -167,12 +167,12 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
         i0 = (int)fmin(k, ns - 1.0);  // s in (n-1, n-0.5] would round to n: clamp to the last sample
     } else if constexpr (INTERP == kLinear) {
         ok &= (unsigned)!(s < 0.0) & (unsigned)!(s > ns - 1.0);
-        const double fl = floor(s);
-        const int i = (int)fl;
-        // s == n-1 exactly: taps (n-2, n-1), t = 1 (n_samples < 2^31: tap indices are 32-bit)
-        const bool last = (i == (int)p.n_samples - 1) & (p.n_samples >= 2);
-        i0 = last ? i - 1 : i;
-        frac = last ? 1.f : (float)(s - fl);
+        // first tap floor(s), clamped to max(n-2, 0), and the fraction taken from the tap: at s == n-1
+        // exactly this gives taps (n-2, n-1) with t = 1, elsewhere floor(s) and s - floor(s) (one record
+        // sample: tap 0, t = 0). n_samples < 2^31: tap indices are 32-bit.
+        const double k = fmin(floor(s), p.lin_tap_max);
+        i0 = (int)k;
+        frac = (float)(s - k);
     } else {
         ok &= (unsigned)!(s < 1.0) & (unsigned)!(s > ns - 2.0);
         const double c = rint(s);
@@ -180,15 +180,14 @@ __device__ __forceinline__ PairSetup pair_time(const DasParams &p, const PairGeo
         frac = (float)(s - c);
     }
     float wc = g.w, ws = 0.f;
-    if (p.rotate) {
-        if constexpr (kFactored) {
-            wc = fmaf(a.c, wb.c, -(a.s * wb.s));
-            ws = fmaf(a.c, wb.s, a.s * wb.c);
-        } else {
-            const Phasor ph = phasor(p.fmod, tau);
-            wc = g.w * ph.c;
-            ws = g.w * ph.s;
-        }
+    if constexpr (kFactored) {
+        // without a rotation the callers pass a = 1 and wb = w: the product is then w and +0 exactly
+        wc = fmaf(a.c, wb.c, -(a.s * wb.s));
+        ws = fmaf(a.c, wb.s, a.s * wb.c);
+    } else if (p.rotate) {
+        const Phasor ph = phasor(p.fmod, tau);
+        wc = g.w * ph.c;
+        ws = g.w * ph.s;
     }
     // Debug builds: every tap of a contributing pair lies in the record (taps tap_step apart; one row
     // when n_samples == 1).
@@ -669,8 +668,9 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 const int64_t e = e0 + lf;
                 PairGeom g{0.0, 0.f};
                 if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
-                // the receive factor of the rotation, with the apodisation weight (transmit chunks)
-                Phasor wb{0.f, 0.f};
+                // the receive factor of the rotation, with the apodisation weight (transmit chunks); the
+                // weight alone without a rotation
+                Phasor wb{g.w, 0.f};
                 if constexpr (kFactor) {
                     if (p.rotate && g.w != 0.f) {
                         const Phasor b = phasor(p.fmod, g.dic);

@@ -57,6 +57,7 @@ struct DasParams {
     double fs;               // sampling frequency
     double fmod;             // modulation frequency (used only when rotate != 0)
     double flat_r2;          // (1 - alpha)^2: the Tukey window's flat top is h^2 <= flat_r2 * R^2
+    double lin_tap_max;      // max(n_samples - 2, 0): the largest first tap of linear interpolation
     float quarter_over_alpha;  // 1 / (4 alpha): taper argument in revolutions per unit of (1 - r)
     int32_t rotate;          // IQ data with fmod != 0
     int32_t accumulate;      // out += result (1) or out = result (0)

@@ -176,6 +176,7 @@ int validate_and_build(const mach_beamform_args *a, mach::DasParams *p) {
     p->fs = (double)a->sampling_freq_hz;
     p->fmod = (double)a->modulation_freq_hz;
     p->flat_r2 = (1.0 - (double)a->tukey_alpha) * (1.0 - (double)a->tukey_alpha);
+    p->lin_tap_max = a->n_samples > 2 ? (double)(a->n_samples - 2) : 0.0;
     p->quarter_over_alpha = a->tukey_alpha > 0.0f ? (float)(0.25 / (double)a->tukey_alpha) : 0.f;
     p->rotate = (a->data_type != MACH_DATA_F32 && a->modulation_freq_hz != 0.0f) ? 1 : 0;
     p->accumulate = a->accumulate ? 1 : 0;
