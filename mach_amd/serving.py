@@ -460,8 +460,8 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     the batch (``plain_min_voxel_frames=None``: never). With
     ``hold_min_points``, large requests wait (at most ``max_hold_s`` while the GPU is busy) for
     same-geometry partners to fuse with. A batch's persistent-kernel submission runs on one of
-    ``n_streams`` compute streams and its direct launches round-robin over all of them, so large
-    requests overlap one another's tails. Returns a ServeReport; with ``keep_outputs`` each request's
+    ``n_streams`` compute streams and its direct launches, largest first, go each to the stream with
+    the least direct work outstanding, so large requests overlap one another's tails. Returns a ServeReport; with ``keep_outputs`` each request's
     ``out`` is a private copy of its result."""
     free, _total = torch.cuda.mem_get_info(device)
     budget = min(int(budget_bytes), int(free * 0.8))
@@ -469,7 +469,8 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
     out_ring = DeviceRing(budget - budget * 3 // 4, device)
     copy_stream = torch.cuda.Stream(device)
     compute = [torch.cuda.Stream(device) for _ in range(max(1, n_streams))]
-    next_direct = 0  # round-robin position of direct launches over the compute streams
+    # direct launches not yet known to be complete, per compute stream: (completion event, points)
+    stream_work = [collections.deque() for _ in compute]
     runner = _batch.BatchRunner(max_requests=max_batch, use_graph=use_graph)
     pending = collections.deque(sorted(reqs, key=lambda r: r.arrival_s))
     waiting = collections.deque()
@@ -571,21 +572,27 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
                 new_marks.append([n_batches, members, ev, None, []])
                 return ev
 
-            # direct launches go round-robin over the compute streams, after the batch stream's work so
-            # far (the copies that fuse groups) and their members' data copies
+            # direct launches, largest first, each on the compute stream with the fewest points of direct
+            # work still outstanding (so the streams' tails end together), after the batch stream's work
+            # so far (the copies that fuse groups) and their members' data copies
             prepared = torch.cuda.Event()
             prepared.record(stream)
             direct_done = []
 
             def run_direct(req):
-                nonlocal next_direct
-                st = compute[next_direct % len(compute)]
-                next_direct += 1
+                for q in stream_work:
+                    while q and q[0][0].query():
+                        q.popleft()
+                i = min(range(len(compute)), key=lambda k: sum(pts for _, pts in stream_work[k]))
+                st = compute[i]
                 if st is not stream:
                     st.wait_event(prepared)
                 _batch.run_plain(req, stream=st)
-                direct_done.append(mark(direct_members[id(req)], on=st))
+                ev = mark(direct_members[id(req)], on=st)
+                direct_done.append(ev)
+                stream_work[i].append((ev, req.points()))
 
+            direct.sort(key=lambda rm: -rm[0].points())
             direct_members = {id(req): members for req, members in direct}
             fails = _submit_isolated(lambda rs: runner.submit(rs, stream=stream), run_direct, batched, direct,
                                      after_batched=lambda: mark([]))
