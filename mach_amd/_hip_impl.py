@@ -166,6 +166,13 @@ def _depth_major(scan: torch.Tensor) -> torch.Tensor:
     return idx[torch.argsort(scan[idx, 2], stable=True)]
 
 
+def _is_grid(dims, n: int) -> bool:
+    """Whether voxels with these per-axis coordinate counts fill most of their rank grid (a grid or a
+    grid with holes), so that tiling it pays; a scattered cloud has about n distinct values per axis,
+    and its rank grid (n^3 cells) is never materialised."""
+    return dims[0] * dims[1] * dims[2] <= 2 * n
+
+
 def _tiled(scan: torch.Tensor, per_wave: int) -> torch.Tensor:
     """Wave-sized voxel tiles. Each axis's coordinates are ranked (exact for grids); voxels are grouped
     into tiles of (depth x lateral) ranks, `per_wave` voxels each, ordered depth-major at the tile
@@ -176,6 +183,8 @@ def _tiled(scan: torch.Tensor, per_wave: int) -> torch.Tensor:
     n = scan.shape[0]
     ranks = [torch.unique(scan[:, a], return_inverse=True)[1] for a in range(3)]
     dims = [int(r.max()) + 1 for r in ranks]
+    if not _is_grid(dims, n):
+        return _depth_major(scan)
     lateral = [a for a in (0, 1) if dims[a] > 1]
     sizes = {2: 1, 0: 1, 1: 1}
     if len(lateral) == 2:
@@ -190,8 +199,8 @@ def _tiled(scan: torch.Tensor, per_wave: int) -> torch.Tensor:
         nt = (dims[a] + sizes[a] - 1) // sizes[a]
         tid = tid * nt + ranks[a] // sizes[a]
         local = local * sizes[a] + ranks[a] % sizes[a]
-    count = torch.bincount(tid)
-    full = count[tid] == per_wave
+    _, inv, count = torch.unique(tid, return_inverse=True, return_counts=True)
+    full = count[inv] == per_wave
     dm_rank = torch.empty(n, dtype=torch.int64, device=scan.device)
     dm_rank[_depth_major(scan)] = torch.arange(n, device=scan.device)
     base = (int(tid.max()) + 1) * per_wave
@@ -224,7 +233,7 @@ def _xcd_wedges(scan: torch.Tensor, per_wave: int, groups: int):
     dev = scan.device
     ranks = [torch.unique(scan[:, a], return_inverse=True)[1] for a in range(3)]
     dims = [int(r.max()) + 1 for r in ranks]
-    if dims[0] < 2 or dims[1] < 2:
+    if dims[0] < 2 or dims[1] < 2 or not _is_grid(dims, n):
         return None
     sz, sx, sy = _VOLUME_TILES[per_wave]
     nz, nx, ny = (dims[2] + sz - 1) // sz, (dims[0] + sx - 1) // sx, (dims[1] + sy - 1) // sy

@@ -125,3 +125,12 @@ def test_xcd_wedges_sectors_are_balanced_on_a_symmetric_grid():
 
 def test_xcd_wedge_order_needs_two_lateral_axes():
     assert _xcd_wedges(grid(64, 1, 64), 64, 4) is None  # planar grids keep their tiles
+
+
+def test_large_scattered_cloud_is_not_tiled():
+    """A scattered cloud has about n distinct coordinates per axis: its rank grid (n^3 cells) is never
+    materialised. The tiled order is then the depth-major one, and there are no XCD wedges."""
+    rng = np.random.default_rng(1)
+    scan = torch.as_tensor((rng.normal(size=(200_000, 3)) * 1e-3).astype(np.float32))
+    assert torch.equal(_tiled(scan, 64), _depth_major(scan))
+    assert _xcd_wedges(scan, 64, 4) is None
