@@ -268,6 +268,9 @@ def _xcd_wedges(scan: torch.Tensor, per_wave: int, groups: int):
     take = np.zeros(_N_XCD, dtype=np.int64)
     seq = []
     h = 0
+    if len(set(left.tolist())) == 1 and left[0] % groups == 0:  # equal sectors: one interleave
+        seq = np.stack(lists).reshape(_N_XCD, -1, groups).transpose(1, 0, 2).reshape(-1)
+        left[:] = 0
     while left.sum():
         for _ in range(groups):
             x = h % _N_XCD if left[h % _N_XCD] else int(np.argmax(left))
@@ -278,7 +281,7 @@ def _xcd_wedges(scan: torch.Tensor, per_wave: int, groups: int):
             left[x] -= 1
         h += 1
     tile_pos = torch.full((nz * nx * ny,), -1, dtype=torch.int64)
-    tile_pos[torch.as_tensor(np.array(seq, dtype=np.int64))] = torch.arange(len(seq))
+    tile_pos[torch.as_tensor(np.asarray(seq, dtype=np.int64))] = torch.arange(len(seq))
     tile_pos = tile_pos.to(dev)[tid]
     dm_rank = torch.empty(n, dtype=torch.int64, device=dev)
     dm_rank[_depth_major(scan)] = torch.arange(n, device=dev)
