@@ -122,10 +122,13 @@ def test_reference_import_name():
 
     assert mach.beamform is mach_amd.beamform
     assert mach.InterpolationType is mach_amd.InterpolationType
-    for name in ("kernel", "experimental", "wavefront", "geometry", "io", "io.must"):
+    for name in ("kernel", "experimental", "wavefront", "geometry", "io", "io.must", "io.uff"):
         mod = importlib.import_module(f"mach.{name}")
         assert mod is importlib.import_module(f"mach_amd.{name}"), name
+    from mach.io.uff import create_beamforming_setup  # the reference's UFF test imports it so
     from mach.kernel import nb_beamform  # the raw entry the reference's benchmarks call
+
+    assert create_beamforming_setup is mach_amd.io.uff.create_beamforming_setup
 
     assert nb_beamform is mach_amd.kernel.nb_beamform
     assert set(["beamform", "geometry", "kernel", "wavefront", "__version__", "Array"]) <= set(mach.__all__)
