@@ -231,7 +231,7 @@ int mach_batch_destroy(mach_batch *batch);
 /* Allocation-free variant of mach_batch_submit, with no counterpart in the reference: for callers that
  * cannot allocate (code under stream or graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). It needs no mach_batch object and allocates nothing, on the host or the device: the
- * descriptor table and work counters are a fixed device array of the library (about 14 KiB, held by
+ * descriptor table and work counters are a fixed device array of the library (about 15 KiB, held by
  * each GPU the library runs on until the process ends), written from kernel arguments, and the same
  * persistent kernels run the requests without graph replay. Results equal mach_batch_submit's and
  * mach_beamform's on each request. At most MACH_NOALLOC_MAX_REQUESTS requests (MACH_ERR_INVALID_ARGUMENT
