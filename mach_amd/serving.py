@@ -487,9 +487,17 @@ def serve_batched(reqs, device, budget_bytes=200 * GIB, max_batch=64, use_graph=
         now = time.perf_counter() - t0
         while pending and pending[0].arrival_s <= now:
             waiting.append(pending.popleft())
-        # admission by HBM budget, in arrival order
+        # admission by HBM budget, in arrival order; a request larger than its ring can never be
+        # admitted, so it fails alone instead of blocking the queue
         while waiting:
             r = waiting[0]
+            if r.data_bytes > data_ring.capacity or r.out_bytes > out_ring.capacity:
+                waiting.popleft()
+                r.error = (f"request needs {r.data_bytes} B of channel data and {r.out_bytes} B of output; the HBM "
+                           f"budget's rings hold {data_ring.capacity} B and {out_ring.capacity} B")
+                r.out, r.admit_s, r.done_s = None, now, now
+                n_done += 1
+                continue
             ds = data_ring.alloc(r.data_bytes)
             if ds is None:
                 break

@@ -106,6 +106,32 @@ def test_a_failing_request_does_not_stop_the_others(device, burst_reqs):
             assert r.error is None and torch.equal(r.out, alone[r.rid]), r.rid
 
 
+def test_a_request_over_the_budget_fails_alone(device, burst_reqs):
+    """A request whose channel data or output is larger than its ring (three quarters of the HBM budget
+    for data, a quarter for outputs) can never be admitted: it completes as failed, with a message, and
+    every other request's result is bitwise its result alone."""
+    reqs = list(burst_reqs)
+    _reset(reqs)
+    serving.serve_plain(reqs, device, keep_outputs=True)
+    alone = {r.rid: r.out for r in reqs}
+    big = max(reqs, key=lambda r: r.data_bytes)
+    budget = int(big.data_bytes * 1.3)  # the data ring holds 0.975x the largest request's data
+    data_cap = (budget * 3 // 4) // 256 * 256
+    out_cap = (budget - budget * 3 // 4) // 256 * 256
+    over = {r.rid for r in reqs if r.data_bytes > data_cap or r.out_bytes > out_cap}
+    assert big.rid in over and len(over) < len(reqs)
+    for r in reqs:
+        r.error = None
+    _reset(reqs)
+    rep = serving.serve_batched(reqs, device, budget_bytes=budget, keep_outputs=True)
+    assert rep.n_requests == len(reqs) and rep.n_failed == len(over)
+    for r in reqs:
+        if r.rid in over:
+            assert r.out is None and "HBM budget" in r.error, r.rid
+        else:
+            assert r.error is None and torch.equal(r.out, alone[r.rid]), r.rid
+
+
 def test_voxel_orders_outlive_the_order_cache(device, monkeypatch):
     """Batches run on the server's own streams and read each request's voxel order there; the
     library's order cache is bounded. With the cache cut to two entries (so orders are dropped from it
