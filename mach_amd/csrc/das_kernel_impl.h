@@ -17,9 +17,14 @@ using namespace dev;
 
 // Waves per SIMD asked of the compiler. The FP16, linear, multi-transmit 1 x 4 x 2 variant (the tuned
 // layout of the 8-frame volume) fits 6 in 80 VGPRs with a few spills, mostly outside its loops: 18.42 ->
-// 18.21 ms against 5 waves at 96 VGPRs.
+// 18.21 ms against 5 waves at 96 VGPRs. The FP16 quadratic 64 x 4 variant (the FP16 Doppler ensemble)
+// fits 6 in 78 VGPRs without spills: power image 0.210 -> 0.207 ms, signal unchanged. (The headline's
+// lateral 4-voxel kernel at 6 waves, 80 VGPRs with spills, measured 0.363 ms against 0.348 at 5.)
 constexpr int das_waves_per_eu(int st, int interp, int lf, int fpl, int ch, int tc) {
-    return (st == kStoreC32H && interp == kLinear && lf == 1 && fpl == 4 && ch == 2 && tc > 1) ? 6 : 1;
+    return ((st == kStoreC32H && interp == kLinear && lf == 1 && fpl == 4 && ch == 2 && tc > 1) ||
+            (st == kStoreC32H && interp == kQuadratic && lf == 64 && fpl == 4 && ch == 1 && tc == 1))
+               ? 6
+               : 1;
 }
 
 template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false, int TSX = 1>
