@@ -13,6 +13,9 @@ data from a fixed seed. One step = one beamform of the whole ensemble into a pre
                (SQ_INSTS_VALU_FMA_F32 x 64 lanes x 2 flop / kernel time) against the FP32 vector peak
   call         the reference's timed closure (tests/compare/test_pymust.py:277-297): `out` zeroed, the
                drop-in entry called (accumulate), and the device synchronised, wall clock per call
+  served       the mixed request stream through the continuous-batching server against one plain launch
+               per request: at 300 QPS (latency; arrival_bound when the plain path keeps up with the
+               arrivals) and offered at 20000 QPS (throughput_ratio: batched vs plain points/s)
   cpu_baseline an FP32 delay-and-sum of the same algorithm (oracle/das_cpu32.c, OpenMP over the host
                cores, built -O3 -march=native on the host it runs on) timed on a seeded voxel sample of
                the same workload; the FP64 oracle's time on the same sample is reported beside it
