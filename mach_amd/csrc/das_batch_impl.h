@@ -64,9 +64,13 @@ __global__ __launch_bounds__(256) void das_batch_kernel(const BatchHeader *__res
                                                         uint32_t *counters) {
     constexpr int kSec = TC > 1 ? 1 : 0;
     __shared__ uint32_t s_work;
-    // transmit factors of multi-transmit chunks and transmit-split partial sums (every variant above
-    // needs at most this much: CH = 1, FPL <= 4 with no split, FPL = 1 with it)
-    __shared__ float lds[TC > 1 ? das_lds_floats<TC, 1, 1, kTxSplitWaves>() : 1];
+    // the LDS of the largest variant above (CH = 1): transmit factors of multi-frame lanes (FPL <= 4, no
+    // split), or the partial sums and shared pair geometry of a transmit split (FPL = 1)
+    constexpr int kLds = TC > 1 ? (das_lds_floats<TC, 1, 4, 1>() > das_lds_floats<TC, 1, 1, kTxSplitWaves>()
+                                       ? das_lds_floats<TC, 1, 4, 1>()
+                                       : das_lds_floats<TC, 1, 1, kTxSplitWaves>())
+                                : 1;
+    __shared__ float lds[kLds];
     const uint32_t n_items = header->n_items[kSec];
     const uint32_t total = header->total_blocks[kSec];
     const BatchItem *__restrict__ items = all_items + header->first_item[kSec];

@@ -539,11 +539,31 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
     return r;
 }
 
-// LDS floats das_block needs: the transmit factors of a chunk (4 waves x TC x 64 lanes, complex) and the
-// partial sums of a transmit split (4 waves x CH x FPL complex rows of 64 lanes); at least 1.
+// Steps of LF elements whose pair geometry a transmit split's workgroup computes together (das_block).
+constexpr int kGeomSteps = 16;
+
+// LDS of das_block, in floats: the transmit factors of a chunk for lanes with several frames (4 waves x
+// TC x 64 lanes, complex), then for a transmit split the partial sums (4 waves x CH x FPL complex rows
+// of 64 lanes) and the shared pair geometry of kGeomSteps element steps (64 lanes each: float64 receive
+// delay, weight, complex receive factor).
+template <int TC, int CH, int FPL>
+constexpr int das_lds_tx_factor_floats() {
+    return (TC > 1 && FPL * CH > 1) ? 4 * TC * 64 * 2 : 0;
+}
+template <int CH, int FPL, int TSX>
+constexpr int das_lds_partial_floats() {
+    return TSX > 1 ? 4 * CH * FPL * 2 * 64 : 0;
+}
+template <int TSX>
+constexpr int das_lds_geom_floats() {
+    return TSX > 1 ? kGeomSteps * 64 * 5 : 0;
+}
+// at least 1
 template <int TC, int CH, int FPL, int TSX>
 constexpr int das_lds_floats() {
-    return (TC > 1 ? 4 * TC * 64 * 2 : 0) + (TSX > 1 ? 4 * CH * FPL * 2 * 64 : 0) + (TC > 1 || TSX > 1 ? 0 : 1);
+    const int n = das_lds_tx_factor_floats<TC, CH, FPL>() + das_lds_partial_floats<CH, FPL, TSX>() +
+                  das_lds_geom_floats<TSX>();
+    return n > 0 ? n : 1;
 }
 
 // One 256-thread block of the delay-and-sum: voxel group `bx` (4 waves x 64/LF voxels, in processing
@@ -554,7 +574,9 @@ constexpr int das_lds_floats() {
 // broadcast and the tap offsets are computed once for all CH groups.
 // Transmits are taken TC at a time: the transmit-independent pair geometry (aperture, receive delay,
 // apodisation) is computed once per chunk and reused for its TC transmits. The per-voxel, per-frame
-// summation order is (transmit chunk, element, transmit within chunk) whatever the layout.
+// summation order is (transmit chunk, element, transmit within chunk) whatever the layout; with a
+// transmit split (TSX > 1) it is (transmit range, element chunk of kGeomSteps * LF elements, transmit
+// chunk, element, transmit within chunk), the ranges added in order at the end.
 //
 // Epilogues (p.output, uniform over the launch):
 //   kOutSignal   - out[v][f] (+)= y, the reference's output.
@@ -574,7 +596,9 @@ constexpr int das_lds_floats() {
 // and read by the same lane, so no barrier): as a register array they cost the 8-frame volume a wave
 // per SIMD (FP16 20.4 -> 22.3 ms).
 // TSX (tx_split_waves of the request: kTxSplitWaves or 1): the workgroup's waves all hold one voxel
-// group, wave w summing transmits [n_tx w / TSX, n_tx (w + 1) / TSX) in chunks of TC; the first wave
+// group, wave w summing transmits [n_tx w / TSX, n_tx (w + 1) / TSX) in chunks of TC. The elements are
+// walked in chunks whose pair geometry the 4 waves first compute together into LDS, each pair once
+// (without the split every wave would compute every pair for each of its transmit chunks); the first wave
 // adds the others' partial sums in wave order through LDS and writes the result. Summation order per
 // voxel: (transmit range, chunk, element, transmit).
 // `lds`: das_lds_floats<TC, CH, FPL, TSX>() floats of the workgroup's LDS.
@@ -626,6 +650,13 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
 #pragma unroll
             for (int k = 0; k < FPL; ++k) acc_re[c][k] = acc_im[c][k] = 0.f;
 
+        // pair geometry shared by a transmit split's waves (LDS table of kGeomSteps element steps)
+        constexpr bool kShareGeom = TSX > 1;
+        float *const geom = lds + das_lds_tx_factor_floats<TC, CH, FPL>() + das_lds_partial_floats<CH, FPL, TSX>();
+        double *const G_dic = (double *)geom;                 // [kGeomSteps][64]
+        float *const G_w = geom + kGeomSteps * 64 * 2;        // [kGeomSteps][64]
+        float2 *const G_wb = (float2 *)(geom + kGeomSteps * 64 * 3);  // [kGeomSteps][64]
+        auto tx_chunks = [&](const int64_t e_lo, const int64_t e_hi) {
         for (int64_t t0 = tx_lo; t0 < tx_hi; t0 += TC) {
             const int nt = (int)((tx_hi - t0) < TC ? (tx_hi - t0) : TC);
             // Transmit arrivals are float32 values (array entries, or wave_arrival's rounding), so they
@@ -664,17 +695,25 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                     return Phasor{1.f, 0.f};
                 }
             };
-            for (int64_t e0 = 0; e0 < p.n_rx; e0 += LF) {
+            for (int64_t e0 = e_lo; e0 < e_hi; e0 += LF) {
                 const int64_t e = e0 + lf;
                 PairGeom g{0.0, 0.f};
-                if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
-                // the receive factor of the rotation, with the apodisation weight (transmit chunks); the
-                // weight alone without a rotation
-                Phasor wb{g.w, 0.f};
-                if constexpr (kFactor) {
-                    if (p.rotate && g.w != 0.f) {
-                        const Phasor b = phasor(p.fmod, g.dic);
-                        wb = Phasor{g.w * b.c, g.w * b.s};
+                Phasor wb{0.f, 0.f};
+                if constexpr (kShareGeom) {
+                    const int at = (int)((e0 - e_lo) / LF) * 64 + lane;
+                    g = PairGeom{G_dic[at], G_w[at]};
+                    const float2 b = G_wb[at];
+                    wb = Phasor{b.x, b.y};
+                } else {
+                    if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+                    // the receive factor of the rotation, with the apodisation weight (transmit chunks);
+                    // the weight alone without a rotation
+                    wb = Phasor{g.w, 0.f};
+                    if constexpr (kFactor) {
+                        if (p.rotate && g.w != 0.f) {
+                            const Phasor b = phasor(p.fmod, g.dic);
+                            wb = Phasor{g.w * b.c, g.w * b.s};
+                        }
                     }
                 }
                 // Multi-transmit chunks whose taps for the whole chunk fit 32 VGPRs (single frames, e.g.
@@ -883,11 +922,42 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
             }
         }
 
+        };
+        if constexpr (kShareGeom) {
+            // element chunks: the 4 waves compute the chunk's kGeomSteps steps of pair geometry together
+            // (wave w steps w * kGeomSteps / 4 ..), the same pairs and values the lanes would compute
+            // themselves, then every wave walks its transmit chunks over them; every wave passes both
+            // barriers, also where it holds no voxel
+            const int64_t e_chunk = (int64_t)kGeomSteps * LF;
+            for (int64_t ec = 0; ec < p.n_rx; ec += e_chunk) {
+#pragma unroll
+                for (int k = 0; k < kGeomSteps / 4; ++k) {
+                    const int st = wave * (kGeomSteps / 4) + k;
+                    const int64_t e = ec + (int64_t)st * LF + lf;
+                    PairGeom g{0.0, 0.f};
+                    if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
+                    Phasor wb{g.w, 0.f};
+                    if (p.rotate && g.w != 0.f) {
+                        const Phasor b = phasor(p.fmod, g.dic);
+                        wb = Phasor{g.w * b.c, g.w * b.s};
+                    }
+                    G_dic[st * 64 + lane] = g.dic;
+                    G_w[st * 64 + lane] = g.w;
+                    G_wb[st * 64 + lane] = make_float2(wb.c, wb.s);
+                }
+                __syncthreads();
+                tx_chunks(ec, ec + e_chunk < p.n_rx ? ec + e_chunk : p.n_rx);
+                __syncthreads();  // the next chunk's geometry overwrites the table
+            }
+        } else {
+            tx_chunks(0, p.n_rx);
+        }
+
         if constexpr (TSX > 1) {
             // the other waves' partial sums, added in wave (transmit) order by the group's first wave;
             // every wave passes both barriers, also where it holds no voxel
             constexpr int kRows = CH * FPL * 2;  // rows of 64 floats per wave
-            float *const part = lds + (TC > 1 ? 4 * TC * 64 * 2 : 0);  // after the transmit factors
+            float *const part = lds + das_lds_tx_factor_floats<TC, CH, FPL>();  // after the transmit factors
 #pragma unroll
             for (int c = 0; c < CH; ++c)
 #pragma unroll

@@ -20,7 +20,8 @@ using namespace dev;
 // 18.21 ms against 5 waves at 96 VGPRs. The FP16 quadratic 64 x 4 variant (the FP16 Doppler ensemble)
 // fits 6 in 78 VGPRs without spills: power image 0.210 -> 0.207 ms, signal unchanged. (The headline's
 // lateral 4-voxel kernel at 6 waves, 80 VGPRs with spills, measured 0.363 ms against 0.348 at 5.)
-constexpr int das_waves_per_eu(int st, int interp, int lf, int fpl, int ch, int tc) {
+constexpr int das_waves_per_eu(int st, int interp, int lf, int fpl, int ch, int tc, int tsx) {
+    if (tsx > 1) return 4;
     return ((st == kStoreC32H && interp == kLinear && lf == 1 && fpl == 4 && ch == 2 && tc > 1) ||
             (st == kStoreC32H && interp == kQuadratic && lf == 64 && fpl == 4 && ch == 1 && tc == 1))
                ? 6
@@ -28,7 +29,7 @@ constexpr int das_waves_per_eu(int st, int interp, int lf, int fpl, int ch, int 
 }
 
 template <int ST, int INTERP, int LF, int FPL, int TC, int CH, bool WIDE = false, int TSX = 1>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(das_waves_per_eu(ST, INTERP, LF, FPL, CH, TC))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(das_waves_per_eu(ST, INTERP, LF, FPL, CH, TC, TSX))))
 void das_kernel(const DasParams p) {
     __shared__ float lds[das_lds_floats<TC, CH, FPL, TSX>()];
     const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
