@@ -540,12 +540,12 @@ __device__ __forceinline__ float emit_power(const DasParams &p, int64_t v, float
 }
 
 // Steps of LF elements whose pair geometry a transmit split's workgroup computes together (das_block).
-constexpr int kGeomSteps = 16;
+constexpr int kGeomSteps = 32;
 
 // LDS of das_block, in floats: the transmit factors of a chunk for lanes with several frames (4 waves x
 // TC x 64 lanes, complex), then for a transmit split the partial sums (4 waves x CH x FPL complex rows
 // of 64 lanes) and the shared pair geometry of kGeomSteps element steps (64 lanes each: float64 receive
-// delay, weight, complex receive factor).
+// delay and the complex receive factor with the weight).
 template <int TC, int CH, int FPL>
 constexpr int das_lds_tx_factor_floats() {
     return (TC > 1 && FPL * CH > 1) ? 4 * TC * 64 * 2 : 0;
@@ -556,7 +556,7 @@ constexpr int das_lds_partial_floats() {
 }
 template <int TSX>
 constexpr int das_lds_geom_floats() {
-    return TSX > 1 ? kGeomSteps * 64 * 5 : 0;
+    return TSX > 1 ? kGeomSteps * 64 * 4 : 0;
 }
 // at least 1
 template <int TC, int CH, int FPL, int TSX>
@@ -653,9 +653,8 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
         // pair geometry shared by a transmit split's waves (LDS table of kGeomSteps element steps)
         constexpr bool kShareGeom = TSX > 1;
         float *const geom = lds + das_lds_tx_factor_floats<TC, CH, FPL>() + das_lds_partial_floats<CH, FPL, TSX>();
-        double *const G_dic = (double *)geom;                 // [kGeomSteps][64]
-        float *const G_w = geom + kGeomSteps * 64 * 2;        // [kGeomSteps][64]
-        float2 *const G_wb = (float2 *)(geom + kGeomSteps * 64 * 3);  // [kGeomSteps][64]
+        double *const G_dic = (double *)geom;                         // [kGeomSteps][64]
+        float2 *const G_wb = (float2 *)(geom + kGeomSteps * 64 * 2);  // [kGeomSteps][64]
         auto tx_chunks = [&](const int64_t e_lo, const int64_t e_hi) {
         for (int64_t t0 = tx_lo; t0 < tx_hi; t0 += TC) {
             const int nt = (int)((tx_hi - t0) < TC ? (tx_hi - t0) : TC);
@@ -700,10 +699,13 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                 PairGeom g{0.0, 0.f};
                 Phasor wb{0.f, 0.f};
                 if constexpr (kShareGeom) {
+                    // the factored timing uses the weight only to skip w == 0, and the receive factor
+                    // w (cos, sin) (or (w, 0) without a rotation) is 0 exactly when w is: w >= 3e-14
+                    // whenever it is not 0, and max(|cos|, |sin|) > 0.7
                     const int at = (int)((e0 - e_lo) / LF) * 64 + lane;
-                    g = PairGeom{G_dic[at], G_w[at]};
                     const float2 b = G_wb[at];
                     wb = Phasor{b.x, b.y};
+                    g = PairGeom{G_dic[at], (b.x != 0.f || b.y != 0.f) ? 1.f : 0.f};
                 } else {
                     if (vok && e < p.n_rx) g = pair_geom(p, xv, yv, zv, R2, inv_R, e);
                     // the receive factor of the rotation, with the apodisation weight (transmit chunks);
@@ -942,7 +944,6 @@ __device__ __forceinline__ void das_block(const DasParams &p, uint32_t bx, uint3
                         wb = Phasor{g.w * b.c, g.w * b.s};
                     }
                     G_dic[st * 64 + lane] = g.dic;
-                    G_w[st * 64 + lane] = g.w;
                     G_wb[st * 64 + lane] = make_float2(wb.c, wb.s);
                 }
                 __syncthreads();
