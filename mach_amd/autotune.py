@@ -270,14 +270,17 @@ def tune_acquisition(acq, storage: str = "c64", reps: int = 20, record: bool = T
     # the band kernel sums transmits in a different order and has no epilogues: keep it to
     # single-transmit signal requests, so that a tuned layout never changes results
     best, times = tune(run, acq.n_frames, reps, band=acq.n_tx == 1 and output == 0, lateral=acq.n_tx == 1)
-    # then the winner's block order over the XCDs (it interacts with the voxel order a layout gets, so it
-    # is chosen per layout: e.g. runs of 8 blocks for the 4x4x4-tiled volume, of 32 for Doppler rows)
-    if times[best] > 0 and (best >> 4) & 0xFF != 0:  # not for the band kernel (its own remap)
-        # XCD wedges only where the layout has wave tiles (the order of the other layouts ignores them)
-        codes = [c for c in XCD_CODES if c != _hip_impl.XCD_WEDGE_CODE or _hip_impl.wave_tile(best) is not None]
-        xbest, xtimes = tune(run, acq.n_frames, reps, cands=[with_xcd(best, c) for c in codes])
-        times.update(xtimes)
-        best = xbest
+    # then the block order over the XCDs of the two fastest layouts (it interacts with the voxel order a
+    # layout gets, so it is chosen per layout: e.g. runs of 8 blocks for the 4x4x4-tiled volume, of 64
+    # for Doppler rows; the runner-up can overtake the winner once both have their best order)
+    if times[best] > 0:
+        bases = [lay for lay in sorted(times, key=times.get) if (lay >> 4) & 0xFF != 0][:2]  # no band kernel
+        for base in bases:
+            # XCD wedges only where the layout has wave tiles (the order of the other layouts ignores them)
+            codes = [c for c in XCD_CODES if c != _hip_impl.XCD_WEDGE_CODE or _hip_impl.wave_tile(base) is not None]
+            _, xtimes = tune(run, acq.n_frames, reps, cands=[with_xcd(base, c) for c in codes])
+            times.update(xtimes)
+        best = min(times, key=times.get)
     key = shape_key(kind, p["interp_type"], acq.n_tx, acq.n_rx, acq.n_frames, acq.n_vox, output)
     if record:
         table().record(key, best, times, {"workload": acq.name, "storage": storage,

@@ -12,7 +12,7 @@ it when ``MACH_AMD_DEBUG=1``. Compiler flags (set in CMakeLists.txt):
     discrete decisions (aperture, bounds, tap index) as a float64 evaluation of the same formulas; the
     kernels spell out every fused multiply-add they want with ``fmaf``.
   * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed-FP32 instructions.
-  * ``-mllvm -structurizecfg-skip-uniform-regions=1`` (complex64 kernels only): branches on
+  * ``-mllvm -structurizecfg-skip-uniform-regions=1`` (complex64 and lateral-group kernels): branches on
     wave-uniform values stay scalar branches, so the lateral kernels' accumulators update in place
     instead of being copied at the end of every element iteration (control flow only).
 """

@@ -149,6 +149,8 @@ bool layout_valid(const DasParams &p, int store, DasLayout l);
 
 // Enqueues the delay-and-sum kernel. Returns hipSuccess or the launch error.
 hipError_t launch_das(const DasParams &p, int store, int interp, DasLayout layout, hipStream_t stream);
+// The lateral-group kernels (layout.voxels_per_lane 2 or 4; das_lateral.hip), reached through launch_das.
+hipError_t launch_das_lateral(const DasParams &p, int store, int interp, DasLayout layout, hipStream_t stream);
 
 #ifdef MACH_DEBUG
 // Debug builds: failed kernel bounds checks since the last read, per translation unit (read and clear).
@@ -158,6 +160,7 @@ hipError_t debug_violations_c32h(uint32_t *count);
 hipError_t debug_violations_batch0(uint32_t *count);
 hipError_t debug_violations_batch1(uint32_t *count);
 hipError_t debug_violations_band(uint32_t *count);
+hipError_t debug_violations_lateral(uint32_t *count);
 #endif
 
 }  // namespace mach

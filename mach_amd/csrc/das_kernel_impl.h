@@ -1,6 +1,7 @@
 // Plain (one request per launch) delay-and-sum kernel templates and their launch dispatch, included by
 // one translation unit per storage type (das_kernel_f32.hip, das_kernel_c64.hip, das_kernel_c32h.hip)
-// so the variants compile in parallel. The device code is in das_device.h.
+// so the variants compile in parallel. The device code is in das_device.h; the lateral-group kernels
+// have their own translation unit (das_lateral.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,7 +9,6 @@
 
 #include "das_device.h"
 #include "das_kernel.h"
-#include "das_lateral.h"
 
 namespace mach {
 namespace {
@@ -36,37 +36,6 @@ void das_kernel(const DasParams p) {
     const uint32_t bx = lb % gridDim.x;
     if ((int64_t)bx * (4 / TSX) * (64 / LF) >= p.n_vox) return;  // row padding (padded_blocks_x)
     das_block<ST, INTERP, LF, FPL, TC, CH, WIDE, TSX>(p, bx, lb / gridDim.x, lds);
-}
-
-template <int ST, int INTERP, int VPL, int FPL>
-__global__ __launch_bounds__(256) void das_lateral_kernel(const DasParams p) {
-    const uint32_t lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, (uint32_t)p.xcd_group);
-    const uint32_t bx = lb % gridDim.x;
-    if ((int64_t)bx * 4 * VPL >= p.n_vox) return;  // row padding (padded_blocks_x)
-    das_lateral_block<ST, INTERP, VPL, FPL>(p, bx, lb / gridDim.x);
-}
-
-// Lateral groups of VPL voxels per wave, 64 * FPL frames per block (single transmit).
-template <int ST, int INTERP, int VPL, int FPL>
-hipError_t launch_lateral(const DasParams &p, hipStream_t stream) {
-    const int64_t n_groups = (p.n_vox + VPL - 1) / VPL;
-    const int64_t per = 64 * FPL;
-    const int64_t blocks_y = p.output == kOutPower ? 1 : (p.n_frames + per - 1) / per;
-    DasParams q = p;
-    q.xcd_group = (int32_t)xcd_group_for(p, (n_groups + 3) / 4, blocks_y);
-    const int64_t blocks_x = padded_blocks_x((n_groups + 3) / 4, blocks_y, (uint32_t)q.xcd_group);
-    if (blocks_x * blocks_y > 0x7fffffff || blocks_y > 65535) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((das_lateral_kernel<ST, INTERP, VPL, FPL>), dim3((unsigned)blocks_x, (unsigned)blocks_y, 1),
-                       dim3(256, 1, 1), 0, stream, q);
-    return hipGetLastError();
-}
-
-template <int ST, int INTERP>
-hipError_t dispatch_lateral(const DasParams &p, DasLayout l, hipStream_t stream) {
-    const bool four = l.frames_per_lane == 4;
-    if (l.voxels_per_lane == 4)
-        return four ? launch_lateral<ST, INTERP, 4, 4>(p, stream) : launch_lateral<ST, INTERP, 4, 2>(p, stream);
-    return four ? launch_lateral<ST, INTERP, 2, 4>(p, stream) : launch_lateral<ST, INTERP, 2, 2>(p, stream);
 }
 
 template <int ST, int INTERP, int LF, int FPL, int CH, bool WIDE = false>
@@ -125,7 +94,7 @@ hipError_t dispatch_fpl(const DasParams &p, int fpl, int ch, hipStream_t stream)
 template <int ST, int INTERP>
 hipError_t dispatch_lf(const DasParams &p, DasLayout l, hipStream_t stream) {
     const int k = l.frames_per_lane, ch = l.chunks_per_lane;
-    if (l.voxels_per_lane != 1) return dispatch_lateral<ST, INTERP>(p, l, stream);
+    if (l.voxels_per_lane != 1) return launch_das_lateral(p, ST, INTERP, l, stream);  // das_lateral.hip
     switch (l.lanes_per_voxel) {
         case 64: return dispatch_fpl<ST, INTERP, 64>(p, k, ch, stream);
         case 32: return dispatch_fpl<ST, INTERP, 32>(p, k, ch, stream);

@@ -252,7 +252,8 @@ static int debug_check(int st, hipStream_t stream, const char *what) {
     uint32_t total = 0;
     for (hipError_t (*read)(uint32_t *) : {mach::debug_violations_f32, mach::debug_violations_c64,
                                            mach::debug_violations_c32h, mach::debug_violations_batch0,
-                                           mach::debug_violations_batch1, mach::debug_violations_band}) {
+                                           mach::debug_violations_batch1, mach::debug_violations_band,
+                                           mach::debug_violations_lateral}) {
         uint32_t c = 0;
         if (e == hipSuccess) e = read(&c);
         total += c;
