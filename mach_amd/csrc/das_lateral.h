@@ -155,6 +155,12 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
                 if (hi - lo <= kSpread && p.n_samples >= kW) {
                     // one window of kW rows, placed to stay inside the record
                     const int start = lo < (int)p.n_samples - kW ? lo : (int)p.n_samples - kW;
+                    // rows the group reads: voxel q reads rows i0[q] - start .. + kN - 1, so hi - start + kN
+                    // of them (at most kW: hi - lo <= kSpread, and a start moved down to n_samples - kW is
+                    // at most kW - kN below hi, whose taps end inside the record). Groups whose tap starts
+                    // coincide or differ by one (most of them) skip the last rows' gathers; the rows left
+                    // unloaded are never read.
+                    const int n_rows = hi - start + kN;
                     const char *rs = r0 + (int64_t)start * step;
                     Rows<ST, kW, FPL> w;
                     // each row: the window's wave-uniform 64-bit base plus a 32-bit lane offset (frame
@@ -165,7 +171,8 @@ __device__ __forceinline__ void das_lateral_block(const DasParams &p, uint32_t b
                     asm volatile("" : "+v"(fo_l));
                     const uint32_t step32 = (uint32_t)step;
 #pragma unroll
-                    for (int r = 0; r < kW; ++r) w.load(r, rs + (uint32_t)(fo_l + (uint32_t)r * step32));
+                    for (int r = 0; r < kW; ++r)
+                        if (r < kN || r < n_rows) w.load(r, rs + (uint32_t)(fo_l + (uint32_t)r * step32));
 #pragma unroll
                     for (int q = 0; q < VPL; ++q) {
                         if (i0[q] < 0) continue;
