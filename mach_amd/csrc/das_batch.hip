@@ -53,7 +53,7 @@ DasLayout batch_layout(const DasParams &p, int store, DasLayout want) {
     // requests whose transmits split the block's waves run as 1 x 1 (the only batched layout compiled
     // with the split; every layout of theirs sums in the same order)
     if (tx_split_waves(p.n_tx, p.n_frames) != 1) return DasLayout{1, 1};
-    if (want.lanes_per_voxel == 0) want = choose_layout(p, store);
+    if (want.lanes_per_voxel == 0) want = choose_general_layout(p, store);
     want.chunks_per_lane = 1;  // the batched kernels walk one frame group per lane
     want.voxels_per_lane = 1;  // and have no lateral groups: a lateral layout runs as its 64-lane layout
     if (layout_index(want) >= 0) return want;

@@ -143,6 +143,9 @@ inline uint32_t xcd_group_for(const DasParams &p, int64_t blocks_x, int64_t bloc
 
 // Picks the layout for a problem (frame count and pointer alignment decide vector widths).
 DasLayout choose_layout(const DasParams &p, int store);
+// The same among the general kernel's layouts (voxels_per_lane 1): for the batched kernels and the band
+// kernel's plan, which have no lateral groups.
+DasLayout choose_general_layout(const DasParams &p, int store);
 // True if `l` is a compiled layout whose frames-per-lane divides n_frames and keeps vector loads and
 // stores aligned for these pointers.
 bool layout_valid(const DasParams &p, int store, DasLayout l);

@@ -227,7 +227,7 @@ static int resolve_plan(const mach_beamform_args *a, const mach::DasParams &p, P
                                                        " (band kernel) does not apply to this request");
         out->band = true;
         out->band_fpl = l.frames_per_lane;
-        out->layout = mach::choose_layout(p, a->data_type);
+        out->layout = mach::choose_general_layout(p, a->data_type);
         return MACH_OK;
     }
     if (!mach::layout_valid(p, a->data_type, l))
